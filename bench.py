@@ -1,0 +1,163 @@
+"""Throughput of the MI355X FDTD step on the 512x384x256 random room.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp32]
+
+One step = one leapfrog update of every cell of the 512x384x256 grid
+(50.3 M cells) plus the boundary-biquad list, the Gaussian-derivative source
+and 8 receivers, with fields already resident in HBM.  Prints one JSON line.
+
+roofline: the stencil launch's algorithmic bytes N*(3s+1) (read p^n and
+p^{n-1}, write p^{n+1}, read one code byte per cell) divided by its mean
+launch time, taken with HIP events around every launch of the timed region
+on the library's own stream.
+cpu_baseline: the CPU oracle (the same scheme, OpenMP over z) on a 128^3
+corner crop of the same room, on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gvoxel-updates/s + % HBM peak; FP16-store vs FP32 at ≤1e-3 IR err vs FP64"
+
+
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def step_bytes(n, nb, nrcv, s):
+    """Algorithmic bytes of one step: N(3s+1) + Nb(69+4s) + R s."""
+    return n * (3 * s + 1) + nb * (69 + 4 * s) + nrcv * s
+
+
+def run_gpu(prep, precision, steps, warmup, zchunks=0):
+    from virs_amd.fdtd import BYTES, setup
+    f = setup(prep, precision, zchunks=zchunks)
+    f.start(warmup + steps)
+    if warmup:
+        f.step(warmup)
+    f.synchronize()
+    t0 = time.perf_counter()
+    total_ms, dense_ms, bnd_ms = f.time_steps(steps)
+    f.synchronize()
+    wall = time.perf_counter() - t0
+    ir = f.receivers(True)
+    mem = f.memory_bytes()
+    f.close()
+    return dict(total_ms=total_ms, dense_ms=dense_ms, bnd_ms=bnd_ms, wall_s=wall, ir=ir,
+                mem=mem, s=BYTES[precision])
+
+
+def cpu_baseline(prep, crop=128, steps=200, warmup=5):
+    from virs_amd import rooms
+    from oracle import oracle as O
+    flags, bidx, bmat = rooms.crop(prep, (crop, crop, crop))
+    cores = host_cores()
+    O.set_threads(cores)
+    sc = prep.scenario
+    n = crop ** 3
+    centre = (crop // 2 * crop + crop // 2) * crop + crop // 2
+    pb = O.Problem(flags, sc.h, dt=prep.dt, bidx=bidx, bmat=bmat, coeffs=sc.coeffs,
+                   soft_idx=[centre], soft_sig=prep.src_sig[:1, : warmup + steps], rcv_idx=[centre])
+    pb.run(warmup, "f32")
+    t0 = time.perf_counter()
+    pb.run(steps, "f32")
+    t = time.perf_counter() - t0
+    # the reference's own CPU step (a different, damped scheme) on the same crop
+    t1 = time.perf_counter()
+    O.refcpu_steps(flags, sc.h, steps)
+    tr = time.perf_counter() - t1
+    return {
+        "value": round(n * steps / t / 1e9, 4), "unit": "Gvoxel-updates/s", "cores": cores,
+        "kind": "port",
+        "sample": f"{crop}^3 corner crop of the 512x384x256 room, {steps} steps after {warmup} "
+                  f"warm-up, FP32 oracle (same scheme, OpenMP over z)",
+        "reference_cpu_step_gvox_s": round(n * steps / tr / 1e9, 4),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "fp64"])
+    ap.add_argument("--variants", default="fp16", help="extra storage precisions to time, comma separated")
+    ap.add_argument("--zchunks", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the FDTD path is measured on a single MI355X "
+              "(multi-GPU z-slab sharding is designed, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    from virs_amd import rooms
+    t0 = time.perf_counter()
+    sc = rooms.random_room()
+    prep = rooms.prepare(sc)
+    print(f"# room built in {time.perf_counter() - t0:.1f}s: dims={prep.room.dims} "
+          f"nb={len(prep.room.bidx)}", file=sys.stderr, flush=True)
+    n = prep.room.n
+    nb = len(prep.room.bidx)
+    nrcv = len(prep.rcv_idx)
+
+    r = run_gpu(prep, args.precision, args.steps, args.warmup, args.zchunks)
+    ms = r["total_ms"] / args.steps
+    value = n * args.steps / (r["total_ms"] / 1e3) / 1e9
+    s = r["s"]
+    dense_bytes = n * (3 * s + 1)
+    achieved = dense_bytes / (r["dense_ms"] / 1e3) / 1e9
+    sbytes = step_bytes(n, nb, nrcv, s)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Gvoxel-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp32": "f32", "fp16": "f16-store/f32", "bf16": "bf16-store/f32", "fp64": "f64"}[args.precision],
+        "data": "synthetic",
+        "config": {"workload": "random room 512x384x256 (6 boxes + 4 tilted slabs, 3 biquads, "
+                               "8 receivers, Gaussian-derivative source)",
+                   "cells": n, "boundary_cells": nb, "storage": args.precision},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_dense", "kernel_ms": round(r["dense_ms"], 5),
+                     "bytes_per_launch": dense_bytes},
+        "step_algorithmic_gbs": round(sbytes / (ms / 1e3) / 1e9, 1),
+        "step_hbm_frac": round(sbytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "boundary_kernel_ms": round(r["bnd_ms"], 5),
+        "device_bytes": r["mem"],
+        "ir_finite": bool(np.isfinite(r["ir"]).all()),
+    }
+    variants = {}
+    for prec in [v for v in args.variants.split(",") if v and v != args.precision]:
+        rv = run_gpu(prep, prec, args.steps, args.warmup, args.zchunks)
+        vv = n * args.steps / (rv["total_ms"] / 1e3) / 1e9
+        db = n * (3 * rv["s"] + 1)
+        variants[prec] = {"value": round(vv, 3), "speedup_vs_main": round(vv / value, 3),
+                          "dense_gbs": round(db / (rv["dense_ms"] / 1e3) / 1e9, 1),
+                          "dense_ms": round(rv["dense_ms"], 5)}
+    if variants:
+        out["variants"] = variants
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(prep)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,150 @@
+/*
+ * virs_fdtd.h -- C ABI of the MI355X (gfx950) FDTD time-stepping library
+ * (libvirs_hip.so).
+ *
+ * The library replaces the reference's GPU backend behind its dispatch layer:
+ *   src/simulator/simulator_dispatch.h:4-25   (16 free functions on VectorSpace*)
+ *   src/simulator/simulator_gpu.h:4-10        (fdtd_gpu_setup / fdtd_gpu_step / ...)
+ * Each entry point below names the reference function it stands in for.
+ * Plain pointers and sizes only; the library owns all device memory, the
+ * caller owns every host buffer.  Every call returns VIRS_OK (0) or a negative
+ * status; virs_last_error() gives the message.  One context = one HIP stream
+ * on one device; contexts share nothing, so several may live in one process
+ * (the reference keeps process-global __constant__ state,
+ * simulator_gpu.cu:21-36,190-212).
+ *
+ * Grid layout (Grid::idx, src/simulator/vectorspace.h:102-105):
+ *   i = z*Ny*Nx + y*Nx + x, x fastest.  Flags u8 (0 air, 1 wall, 2 boundary
+ *   shell, 3 hard source), boundary list u32 ascending.
+ */
+#ifndef VIRS_FDTD_H
+#define VIRS_FDTD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VIRS_OK 0
+#define VIRS_ERR_ARG -1
+#define VIRS_ERR_HIP -2
+#define VIRS_ERR_STATE -3
+#define VIRS_ERR_NOMEM -4
+#define VIRS_ERR_UNSUPPORTED -5
+
+/* Storage precision of the pressure fields.  FP16/BF16 store p^n and p^{n-1}
+ * with a per-field power-of-two scale and compute in FP32; FP64 is the
+ * reference-scheme oracle.  Boundary-filter state is FP64 in every mode. */
+#define VIRS_FP32 0
+#define VIRS_FP16 1
+#define VIRS_BF16 2
+#define VIRS_FP64 3
+
+#define VIRS_MAX_SPONGE 64
+#define VIRS_MAX_POINTS 64
+
+typedef struct virs_ctx virs_ctx;
+
+typedef struct {
+    int64_t nx, ny, nz;     /* grid dimensions (cells) */
+    float h;                /* cell size in metres */
+    float c;                /* speed of sound, 343 in the reference */
+    float dt;               /* time step in seconds; 0 = the reference's
+                               0.5f*h/(c*sqrtf(3.f)) (simulator_gpu.cu:196) */
+    int32_t precision;      /* VIRS_FP32 / FP16 / BF16 / FP64 */
+    int32_t sponge_width;   /* graded damping layer at the domain edge, cells; 0 = none */
+    float sponge_sigma;     /* damping per step at the outermost cell */
+    int32_t device;         /* HIP device ordinal */
+    int32_t use_graph;      /* replay batches of steps as HIP graphs */
+    int32_t zchunks;        /* z-march chunks of the stencil launch; 0 = automatic */
+} virs_desc;
+
+/* Fills the reference defaults: c = 343, dt = 0, FP32, no sponge, device 0. */
+int virs_desc_default(virs_desc *d);
+
+/* fdtd_gpu_setup (simulator_gpu.cpp:7-235): allocate the device grid, zeroed. */
+int virs_create(const virs_desc *d, virs_ctx **out);
+/* fdtd_gpu_cleanup (simulator_gpu.cpp:237-305). */
+int virs_destroy(virs_ctx *ctx);
+const char *virs_last_error(const virs_ctx *ctx);
+
+/* Reference constants actually used by the kernels (float, computed as
+ * simulator_gpu.cu:195-198,147): dt, c*c*dt*dt, 1/(h*h), (c*dt)/h*0.95f. */
+int virs_get_constants(const virs_ctx *ctx, float *dt, float *c2dt2, float *ih2, float *kconst);
+
+/* updateGPUFromGrid, flags part (simulator_dispatch.cpp:70-81).  Values > 3
+ * are rejected. */
+int virs_upload_flags(virs_ctx *ctx, const uint8_t *flags);
+/* uploadNormalsToGPU (simulator_dispatch.cpp:102-109).  Kept on the device
+ * as in the reference; no kernel of the step reads it. */
+int virs_upload_normals(virs_ctx *ctx, const uint8_t *normals);
+/* uploadBoundaryIndicesToGPU + allocFilterStates (simulator_dispatch.cpp:120-160).
+ * mat may be NULL (every entry uses material row 0, as the reference does).
+ * Filter state is zeroed. */
+int virs_upload_boundary(virs_ctx *ctx, const uint32_t *idx, const uint8_t *mat, int64_t nb);
+/* allocFilterCoeffs + uploadFilterCoeffsToGPU (simulator_dispatch.cpp:162-280):
+ * coeffs[nmat][5] = b0, b1, b2, a1, a2 (a0 = 1). */
+int virs_set_materials(virs_ctx *ctx, const double *coeffs, int32_t nmat);
+
+/* Flag-3 (hard) source value per step.  sig == NULL restores the reference
+ * pulse: 10 for steps 0-9, then 0 (simulator_gpu.cu:214-218). */
+int virs_set_hard_source(virs_ctx *ctx, const double *sig, int64_t len);
+/* Soft (additive) point sources, no reference counterpart: after the
+ * stencil, p[idx[k]] += sig[k][n].  Cells must be distinct. */
+int virs_set_soft_sources(virs_ctx *ctx, const int64_t *idx, const double *sig, int32_t nsrc,
+                          int64_t len);
+/* Receiver voxels.  The reference has one, fixed at (334,225,270)
+ * (simulator_gpu.cu:295). */
+int virs_set_receivers(virs_ctx *ctx, const int64_t *idx, int32_t nrcv);
+/* RMS early stop of simulator_gpu.cu:303-330 on receiver 0; off by default. */
+int virs_set_early_stop(virs_ctx *ctx, int32_t enable, float target_rms);
+
+/* fdtd_start_simulation (simulator_dispatch.cpp:93-100): size the receiver
+ * buffer for nsteps, zero fields, filter state and step counter. */
+int virs_start_simulation(virs_ctx *ctx, int64_t nsteps);
+/* nsteps calls of fdtd_gpu_step (simulator_gpu.cu:184-332), launched without
+ * host round trips.  *keep_going (may be NULL) is 0 once the RMS rule fired;
+ * the steps after it are not counted in virs_steps_done. */
+int virs_step(virs_ctx *ctx, int64_t nsteps, int32_t *keep_going);
+int64_t virs_steps_done(const virs_ctx *ctx);
+int virs_synchronize(virs_ctx *ctx);
+/* Grid::percent_to_target_RMS of the last step (vectorspace.h:98). */
+float virs_percent_to_target_rms(const virs_ctx *ctx);
+
+/* Grid::p_audio_output for every receiver: out[r][k] for k < steps_done. */
+int virs_read_receivers(virs_ctx *ctx, float *out);
+int virs_read_receivers_f64(virs_ctx *ctx, double *out);
+/* updateCurrentGridFromGPU / updateAllGridFromGPU (simulator_dispatch.cpp:31-49):
+ * which = 0 current field p^n, 1 previous p^{n-1}; values in physical units. */
+int virs_read_field(virs_ctx *ctx, int32_t which, float *out);
+int virs_read_field_f64(virs_ctx *ctx, int32_t which, double *out);
+/* updateGPUFromGrid, field part: set p^n (which 0) or p^{n-1} (which 1). */
+int virs_write_field_f64(virs_ctx *ctx, int32_t which, const double *in);
+/* Power-of-two exponent e of a stored field (stored = value * 2^e); 0 unless FP16/BF16. */
+int virs_field_exponent(virs_ctx *ctx, int32_t which, int32_t *e);
+
+/* Device bytes held by the context. */
+int virs_memory_bytes(const virs_ctx *ctx, int64_t *bytes);
+
+/* Runs nsteps and times them on the context's stream with HIP events:
+ * total_ms over the whole run, and the mean duration of the stencil launch
+ * and of the boundary/point launch (events around every launch). */
+int virs_time_steps(virs_ctx *ctx, int64_t nsteps, double *total_ms, double *dense_ms,
+                    double *boundary_ms);
+
+/* Adjoint gradient of an IR loss with respect to every material's
+ * (b0,b1,b2,a1,a2); no reference counterpart.
+ * virs_forward_record runs nsteps from zero state keeping ncheckpoints
+ * snapshots (p^n, p^{n-1}, filter state); the IR is then readable as usual.
+ * virs_adjoint takes dL/dIR [nrcv][nsteps] and writes grad[nmat][5]; it
+ * recomputes each segment forward from its snapshot and runs the
+ * time-reversed stencil with the transposed boundary filters.  FP32/FP64. */
+int virs_forward_record(virs_ctx *ctx, int64_t nsteps, int32_t ncheckpoints);
+int virs_adjoint(virs_ctx *ctx, const float *dl_dir, double *grad);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

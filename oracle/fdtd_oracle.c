@@ -1,0 +1,377 @@
+/*
+ * fdtd_oracle.c -- CPU restatement of the VIRS FDTD time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP path: only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ * Nothing in the product (virs_amd/) links or calls it.
+ *
+ * What it restates (reference file:line):
+ *   oracle_run_f32 / oracle_run_f64
+ *     dense update      src/simulator/simulator_gpu.cu:38-103   (fdtd_kernel)
+ *     boundary biquad   src/simulator/simulator_gpu.cu:105-150  (fdtd_kernel_boundary_biquad)
+ *     hard source       src/simulator/simulator_gpu.cu:59-63,214-218
+ *     rotate + receiver src/simulator/simulator_gpu.cu:292-300
+ *     constants         src/simulator/simulator_gpu.cu:195-198,147
+ *   Extensions of this build (no reference counterpart), each reducing to the
+ *   reference when switched off: per-voxel material rows, FP64 filter state,
+ *   soft point sources, several receivers, graded sponge damping.
+ *   oracle_rms_continue   src/simulator/simulator_gpu.cu:303-330
+ *   oracle_refcpu_steps   src/simulator/simulator_cpu.cpp:55-81 with
+ *                         Grid::laplacian7 src/simulator/vectorspace.h:107-134
+ *   oracle_adjoint_f64    reverse-mode transpose of oracle_run_f64 (no reference
+ *                         counterpart), written as a literal scatter-form
+ *                         transpose of the forward code.
+ *
+ * Pinning (tests/test_oracle.py): rigid-box discrete eigenmodes, discrete
+ * energy conservation, source/receiver reciprocity, scipy.signal.lfilter for
+ * the DF1 recursion, a single-cell hand derivation of the boundary correction,
+ * central finite differences for the adjoint, brute-force numpy steps on tiny
+ * grids for the float32 arithmetic order, and a literal-formula check of the
+ * RMS stop on synthetic IRs.
+ *
+ * Built with -ffp-contract=off so every multiply and add rounds separately.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct {
+    int32_t nx, ny, nz;
+    /* Reference constants, computed by the caller in float exactly as
+     * simulator_gpu.cu:195-198 and :147 do: c2dt2 = c*c*dt*dt,
+     * ih2 = 1/(h*h), kconst = (c*dt)/h*0.95f.  The FP64 runner promotes these
+     * same float values, so FP32 and FP64 runs share one discrete scheme. */
+    float c2dt2, ih2, kconst;
+    const uint8_t *flags;           /* [N] 0 air, 1 wall, 2 boundary shell, 3 hard source */
+    int32_t sponge_w;               /* width of the graded damping layer, 0 = off */
+    const float *spg_a, *spg_b, *spg_g; /* [sponge_w], indexed by distance to the domain edge */
+    int64_t nb;                     /* boundary list length */
+    const uint32_t *bidx;           /* [nb] ascending linear indices of flag-2 cells */
+    const uint8_t *bmat;            /* [nb] material row, NULL = row 0 (the reference) */
+    int32_t nmat;
+    const double *coeffs;           /* [nmat][5] = b0 b1 b2 a1 a2 */
+    int32_t state_f32;              /* 1: reference float state + float coefficients */
+    const double *hard_sig;         /* flag-3 value per step, NULL = reference pulse */
+    int64_t hard_len;
+    int32_t nsoft;
+    const int64_t *soft_idx;        /* [nsoft] */
+    const double *soft_sig;         /* [nsoft][soft_len] */
+    int64_t soft_len;
+    int32_t nrcv;
+    const int64_t *rcv_idx;         /* [nrcv] */
+    const double *init_p, *init_q;  /* optional initial p^0 and p^-1, NULL = zero */
+} oracle_desc;
+
+static int g_threads = 1;
+
+void oracle_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+int oracle_get_threads(void) { return g_threads; }
+
+/* Hard-source value at step n: simulator_gpu.cu:214-218 (10 for n<10, else 0). */
+static double hard_value(const oracle_desc *d, int64_t n)
+{
+    if (!d->hard_sig)
+        return n < 10 ? 10.0 : 0.0;
+    return (n >= 0 && n < d->hard_len) ? d->hard_sig[n] : 0.0;
+}
+
+static int sponge_dist(const oracle_desc *d, int x, int y, int z)
+{
+    int m = x;
+    if (d->nx - 1 - x < m) m = d->nx - 1 - x;
+    if (y < m) m = y;
+    if (d->ny - 1 - y < m) m = d->ny - 1 - y;
+    if (z < m) m = z;
+    if (d->nz - 1 - z < m) m = d->nz - 1 - z;
+    return m;
+}
+
+/*
+ * One generic runner per precision.  T is the field type; the arithmetic of
+ * the dense update is done in T, in the reference's operand order:
+ *   lap = (V(x+1)+V(x-1)+V(y+1)+V(y-1)+V(z+1)+V(z-1) - 6*pc) * ih2
+ *   R   = a*pc - b*Q + g*(c2dt2*lap)        (a,b,g = 2,1,1 outside the sponge)
+ * V(n) = pc if flags[n]==1 else P[n]                     (simulator_gpu.cu:70-81)
+ * The boundary correction is evaluated in double as the reference's literal
+ * 0.1/0.9 promote it (simulator_gpu.cu:147-148).
+ */
+#define DEFINE_RUNNER(T, SUFFIX)                                                          \
+int oracle_run_##SUFFIX(const oracle_desc *d, int64_t n0, int64_t nsteps, double *ir,      \
+                        double *final_p, double *final_q, double *trace_d,                 \
+                        double *trace_y)                                                   \
+{                                                                                         \
+    const int nx = d->nx, ny = d->ny, nz = d->nz;                                         \
+    const int64_t sxy = (int64_t)nx * ny;                                                 \
+    const int64_t N = sxy * nz;                                                           \
+    T *P = (T *)calloc((size_t)N, sizeof(T));                                             \
+    T *Q = (T *)calloc((size_t)N, sizeof(T));                                             \
+    T *R = (T *)calloc((size_t)N, sizeof(T));                                             \
+    double *st = (double *)calloc((size_t)(d->nb > 0 ? d->nb : 1) * 4, sizeof(double));   \
+    float *stf = (float *)calloc((size_t)(d->nb > 0 ? d->nb : 1) * 4, sizeof(float));     \
+    if (!P || !Q || !R || !st || !stf) {                                                  \
+        free(P); free(Q); free(R); free(st); free(stf);                                   \
+        return -1;                                                                        \
+    }                                                                                     \
+    if (d->init_p) for (int64_t i = 0; i < N; ++i) P[i] = (T)d->init_p[i];                \
+    if (d->init_q) for (int64_t i = 0; i < N; ++i) Q[i] = (T)d->init_q[i];                \
+    const T c2dt2 = (T)d->c2dt2, ih2 = (T)d->ih2;                                         \
+    for (int64_t s = 0; s < nsteps; ++s) {                                                \
+        const int64_t n = n0 + s;                                                         \
+        const T src = (T)hard_value(d, n);                                                \
+        _Pragma("omp parallel for num_threads(g_threads) schedule(static)")               \
+        for (int z = 1; z < nz - 1; ++z) {                                                \
+            for (int y = 1; y < ny - 1; ++y) {                                            \
+                for (int x = 1; x < nx - 1; ++x) {                                        \
+                    const int64_t i = (int64_t)z * sxy + (int64_t)y * nx + x;             \
+                    const uint8_t f = d->flags[i];                                        \
+                    if (f == 1) { R[i] = (T)0; continue; }                                \
+                    if (f == 3) { R[i] = src; continue; }                                 \
+                    if (f != 0 && f != 2) continue;                                       \
+                    const T pc = P[i];                                                    \
+                    const T v0 = d->flags[i + 1] == 1 ? pc : P[i + 1];                    \
+                    const T v1 = d->flags[i - 1] == 1 ? pc : P[i - 1];                    \
+                    const T v2 = d->flags[i + nx] == 1 ? pc : P[i + nx];                  \
+                    const T v3 = d->flags[i - nx] == 1 ? pc : P[i - nx];                  \
+                    const T v4 = d->flags[i + sxy] == 1 ? pc : P[i + sxy];                \
+                    const T v5 = d->flags[i - sxy] == 1 ? pc : P[i - sxy];                \
+                    T sum = v0 + v1;                                                      \
+                    sum = sum + v2;                                                       \
+                    sum = sum + v3;                                                       \
+                    sum = sum + v4;                                                       \
+                    sum = sum + v5;                                                       \
+                    const T lap = (sum - (T)6 * pc) * ih2;                                \
+                    T a = (T)2, b = (T)1, g = (T)1;                                       \
+                    if (d->sponge_w > 0) {                                                \
+                        const int dd = sponge_dist(d, x, y, z);                           \
+                        if (dd < d->sponge_w) {                                           \
+                            a = (T)d->spg_a[dd]; b = (T)d->spg_b[dd]; g = (T)d->spg_g[dd]; \
+                        }                                                                 \
+                    }                                                                     \
+                    R[i] = (a * pc - b * Q[i]) + g * (c2dt2 * lap);                       \
+                }                                                                         \
+            }                                                                             \
+        }                                                                                 \
+        for (int k = 0; k < d->nsoft; ++k) {                                              \
+            const int64_t si = d->soft_idx[k];                                            \
+            const double v = n < d->soft_len ? d->soft_sig[(int64_t)k * d->soft_len + n] : 0.0; \
+            R[si] = R[si] + (T)v;                                                         \
+        }                                                                                 \
+        for (int64_t e = 0; e < d->nb; ++e) {                                             \
+            const int64_t bi = d->bidx[e];                                                \
+            const int m = d->bmat ? d->bmat[e] : 0;                                       \
+            const double *c = d->coeffs + 5 * m;                                          \
+            const T dif = P[bi] - Q[bi];                                                  \
+            double yv;                                                                    \
+            if (d->state_f32) {                                                           \
+                float *s4 = stf + 4 * e;                                                  \
+                const float in = (float)dif;                                              \
+                const float o = (float)c[0] * in + (float)c[1] * s4[0] + (float)c[2] * s4[1] \
+                                - (float)c[3] * s4[2] - (float)c[4] * s4[3];              \
+                s4[1] = s4[0]; s4[0] = in; s4[3] = s4[2]; s4[2] = o;                      \
+                yv = (double)o;                                                           \
+            } else {                                                                      \
+                double *s4 = st + 4 * e;                                                  \
+                const double in = (double)dif;                                            \
+                const double o = c[0] * in + c[1] * s4[0] + c[2] * s4[1]                  \
+                                 - c[3] * s4[2] - c[4] * s4[3];                           \
+                s4[1] = s4[0]; s4[0] = in; s4[3] = s4[2]; s4[2] = o;                      \
+                yv = o;                                                                   \
+            }                                                                             \
+            if (trace_d) trace_d[s * d->nb + e] = (double)dif;                            \
+            if (trace_y) trace_y[s * d->nb + e] = yv;                                     \
+            R[bi] = (T)((double)R[bi] -                                                   \
+                        (double)d->kconst * (yv * 0.1 + (double)dif * 0.9) * (double)0.4f); \
+        }                                                                                 \
+        T *t = Q; Q = P; P = R; R = t;                                                    \
+        for (int r = 0; r < d->nrcv; ++r) ir[(int64_t)r * nsteps + s] = (double)P[d->rcv_idx[r]]; \
+    }                                                                                     \
+    if (final_p) for (int64_t i = 0; i < N; ++i) final_p[i] = (double)P[i];               \
+    if (final_q) for (int64_t i = 0; i < N; ++i) final_q[i] = (double)Q[i];               \
+    free(P); free(Q); free(R); free(st); free(stf);                                       \
+    return 0;                                                                             \
+}
+
+DEFINE_RUNNER(float, f32)
+DEFINE_RUNNER(double, f64)
+
+/*
+ * RMS early stop, simulator_gpu.cu:303-330, with the reference's float
+ * arithmetic.  ir[0..step] holds the receiver samples so far; *state is
+ * Grid::rms_filter_state and *percent Grid::percent_to_target_RMS.
+ * Returns 1 to continue, 0 to stop.
+ */
+int oracle_rms_continue(const float *ir, uint32_t step, float *state, float *percent,
+                        float target_rms)
+{
+    const uint32_t RMS_SIZE = 100, MIN_STEP = 3000;
+    if (step > MIN_STEP) {
+        float sum = 0.0f;
+        for (uint32_t i = step - RMS_SIZE; i < step; ++i)
+            sum += ir[i] * ir[i];
+        float rms = sqrtf(sum / (float)RMS_SIZE);
+        if (step == MIN_STEP + 1) {
+            *state = rms;
+        } else {
+            const float fg = 0.00008f;
+            const float vout = (fg * rms + *state) / (fg + 1.f);
+            *state = 2.f * vout - *state;
+            rms = vout;
+        }
+        *percent = target_rms / rms;
+        if (rms < target_rms)
+            return 0;
+    }
+    return 1;
+}
+
+/*
+ * The reference's own CPU step (a different scheme from the GPU one):
+ * pressure-release walls via Grid::laplacian7 and uniform damping gamma = 5,
+ * simulator_cpu.cpp:55-81, vectorspace.h:107-134.  tbb::parallel_for over z
+ * becomes an OpenMP loop over z.  Buffers rotate as in :79-80; on return
+ * *P_io holds the newest field.
+ */
+void oracle_refcpu_steps(int32_t nx, int32_t ny, int32_t nz, const uint8_t *flags, float h,
+                         int32_t nsteps, float **P_io, float **Q_io, float **R_io)
+{
+    const float inv_h2 = 1.f / (h * h);
+    const float c = 343.f;
+    const float dt = (float)(0.5 * (double)h / (double)(c * sqrtf(3.f)));
+    const float c2_dt2 = c * c * dt * dt;
+    const float gamma = 5.f;
+    const float damp = gamma * dt;
+    const int64_t sxy = (int64_t)nx * ny;
+    float *P = *P_io, *Q = *Q_io, *R = *R_io;
+    for (int s = 0; s < nsteps; ++s) {
+        _Pragma("omp parallel for num_threads(g_threads) schedule(static)")
+        for (int k = 1; k < nz - 1; ++k) {
+            for (int j = 1; j < ny - 1; ++j) {
+                for (int i = 1; i < nx - 1; ++i) {
+                    const int64_t idx = (int64_t)k * sxy + (int64_t)j * nx + i;
+                    float lap = 0.0f;
+                    if (flags[idx] != 1) {
+                        const float pc = P[idx];
+                        lap = (P[idx + 1] + P[idx - 1] + P[idx + nx] + P[idx - nx] +
+                               P[idx + sxy] + P[idx - sxy] - 6.0f * pc) * inv_h2;
+                    }
+                    R[idx] = (2.f - damp) * P[idx] - (1.f - damp) * Q[idx] + c2_dt2 * lap;
+                }
+            }
+        }
+        float *t = Q; Q = P; P = R; R = t;
+    }
+    *P_io = P; *Q_io = Q; *R_io = R;
+}
+
+/*
+ * Reverse-mode gradient of L(IR) with respect to every material's
+ * (b0,b1,b2,a1,a2), for the FP64 scheme of oracle_run_f64, from zero initial
+ * fields.  dL_dir is [nrcv][nsteps].  grad is [nmat][5], overwritten.
+ * Written as the literal transpose of the forward code, statement by
+ * statement in reverse, scattering into the adjoints of the inputs.  Keeps
+ * the whole forward filter trajectory in memory: small grids only.
+ */
+int oracle_adjoint_f64(const oracle_desc *d, int64_t nsteps, const double *dL_dir,
+                       double *grad)
+{
+    const int nx = d->nx, ny = d->ny, nz = d->nz;
+    const int64_t sxy = (int64_t)nx * ny, N = sxy * nz, nb = d->nb;
+    const double lam = (double)d->c2dt2 * (double)d->ih2; /* g * c2dt2 * (.. * ih2) */
+    const double c2dt2 = d->c2dt2, ih2 = d->ih2;
+    double *tr_d = (double *)malloc(sizeof(double) * (size_t)(nsteps * (nb > 0 ? nb : 1)));
+    double *tr_y = (double *)malloc(sizeof(double) * (size_t)(nsteps * (nb > 0 ? nb : 1)));
+    double *ir = (double *)malloc(sizeof(double) * (size_t)(nsteps * (d->nrcv > 0 ? d->nrcv : 1)));
+    double *ub_next = (double *)calloc((size_t)N, sizeof(double)); /* adjoint of u^{n+1} */
+    double *ub_cur = (double *)calloc((size_t)N, sizeof(double));  /* adjoint of u^{n}   */
+    double *ub_prev = (double *)calloc((size_t)N, sizeof(double)); /* adjoint of u^{n-1} */
+    double *sadj = (double *)calloc((size_t)(nb > 0 ? nb : 1) * 4, sizeof(double));
+    (void)lam;
+    if (!tr_d || !tr_y || !ir || !ub_next || !ub_cur || !ub_prev || !sadj) {
+        free(tr_d); free(tr_y); free(ir); free(ub_next); free(ub_cur); free(ub_prev); free(sadj);
+        return -1;
+    }
+    if (d->state_f32 || d->init_p || d->init_q) {
+        free(tr_d); free(tr_y); free(ir); free(ub_next); free(ub_cur); free(ub_prev); free(sadj);
+        return -2;
+    }
+    oracle_run_f64(d, 0, nsteps, ir, NULL, NULL, tr_d, tr_y);
+    memset(grad, 0, sizeof(double) * 5 * (size_t)d->nmat);
+    const double kap = (double)d->kconst * (double)0.4f;
+    for (int64_t s = nsteps - 1; s >= 0; --s) {
+        /* IR[s] = u^{s+1}[rcv] */
+        for (int r = 0; r < d->nrcv; ++r)
+            ub_next[d->rcv_idx[r]] += dL_dir[(int64_t)r * nsteps + s];
+        /* boundary correction and filter, reversed */
+        for (int64_t e = nb - 1; e >= 0; --e) {
+            const int64_t bi = d->bidx[e];
+            const int m = d->bmat ? d->bmat[e] : 0;
+            const double *c = d->coeffs + 5 * m;
+            const double rb = ub_next[bi];
+            /* R_b = R_dense - K*(0.1*y + 0.9*dif)*0.4 */
+            double ybar = -0.1 * kap * rb;
+            double dbar = -0.9 * kap * rb;
+            /* state after step s: (x1,x2,y1,y2) = (dif, x1_old, y, y1_old) */
+            double *sa = sadj + 4 * e;
+            dbar += sa[0];
+            double x1bar = sa[1];
+            ybar += sa[2];
+            double y1bar = sa[3];
+            double x2bar = 0.0, y2bar = 0.0;
+            /* y = b0*dif + b1*x1 + b2*x2 - a1*y1 - a2*y2 */
+            const double dv = tr_d[s * nb + e];
+            const double x1 = s >= 1 ? tr_d[(s - 1) * nb + e] : 0.0;
+            const double x2 = s >= 2 ? tr_d[(s - 2) * nb + e] : 0.0;
+            const double y1 = s >= 1 ? tr_y[(s - 1) * nb + e] : 0.0;
+            const double y2 = s >= 2 ? tr_y[(s - 2) * nb + e] : 0.0;
+            grad[5 * m + 0] += ybar * dv;
+            grad[5 * m + 1] += ybar * x1;
+            grad[5 * m + 2] += ybar * x2;
+            grad[5 * m + 3] += -ybar * y1;
+            grad[5 * m + 4] += -ybar * y2;
+            dbar += c[0] * ybar;
+            x1bar += c[1] * ybar;
+            x2bar += c[2] * ybar;
+            y1bar += -c[3] * ybar;
+            y2bar += -c[4] * ybar;
+            sa[0] = x1bar; sa[1] = x2bar; sa[2] = y1bar; sa[3] = y2bar;
+            /* dif = P[b] - Q[b] */
+            ub_cur[bi] += dbar;
+            ub_prev[bi] -= dbar;
+        }
+        /* dense update, reversed: R_i = a*P_i - b*Q_i + g*(c2dt2*((sum V - 6 P_i)*ih2)) */
+        for (int z = 1; z < nz - 1; ++z)
+            for (int y = 1; y < ny - 1; ++y)
+                for (int x = 1; x < nx - 1; ++x) {
+                    const int64_t i = (int64_t)z * sxy + (int64_t)y * nx + x;
+                    const uint8_t f = d->flags[i];
+                    if (f != 0 && f != 2) continue;
+                    const double rb = ub_next[i];
+                    if (rb == 0.0) continue;
+                    double a = 2.0, b = 1.0, g = 1.0;
+                    if (d->sponge_w > 0) {
+                        const int dd = sponge_dist(d, x, y, z);
+                        if (dd < d->sponge_w) { a = d->spg_a[dd]; b = d->spg_b[dd]; g = d->spg_g[dd]; }
+                    }
+                    const double lbar = g * c2dt2 * ih2 * rb; /* adjoint of (sum V - 6 pc) */
+                    ub_cur[i] += a * rb - 6.0 * lbar;
+                    ub_prev[i] += -b * rb;
+                    const int64_t nbr[6] = {i + 1, i - 1, i + nx, i - nx, i + sxy, i - sxy};
+                    for (int k = 0; k < 6; ++k) {
+                        if (d->flags[nbr[k]] == 1) ub_cur[i] += lbar;
+                        else ub_cur[nbr[k]] += lbar;
+                    }
+                }
+        /* shift: the adjoint of u^{s} becomes the one of the next output */
+        double *t = ub_next;
+        ub_next = ub_cur;
+        ub_cur = ub_prev;
+        ub_prev = t;
+        memset(ub_prev, 0, sizeof(double) * (size_t)N);
+    }
+    free(tr_d); free(tr_y); free(ir); free(ub_next); free(ub_cur); free(ub_prev); free(sadj);
+    return 0;
+}

@@ -1,0 +1,146 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Bar: FP32 and FP64 bit-exact against the oracle restatement of the same
+scheme (both sides evaluate the same expressions without FP contraction);
+FP32 against the reference-literal oracle (FP32 filter state, the
+reference's own arithmetic) within 1e-4 relative L2 over the whole IR.
+"""
+import numpy as np
+import pytest
+
+from helpers import gpu_context, oracle_problem, rel_l2, tiny_problem
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_tiny_bit_exact(gpu, seed, precision):
+    p = tiny_problem(seed)
+    _say(f"tiny seed={seed} {precision} dims={p['dims']} nb={len(p['bidx'])}")
+    f = gpu_context(p, precision)
+    f.step(p["steps"])
+    f.synchronize()
+    ir = f.receivers(f64=True)
+    fld = f.field(0)
+    prev = f.field(1)
+    ref = oracle_problem(p).run(p["steps"], "f32" if precision == "fp32" else "f64", want_fields=True)
+    np.testing.assert_array_equal(ir, ref["ir"])
+    np.testing.assert_array_equal(fld, ref["p"])
+    np.testing.assert_array_equal(prev, ref["q"])
+    f.close()
+
+
+def test_constants_match_oracle(gpu):
+    p = tiny_problem(0)
+    f = gpu_context(p)
+    dt, c2dt2, ih2, k = f.constants
+    odt, oc, oi, ok = O.reference_constants(p["h"], dt=p["dt"])
+    assert (dt, c2dt2, ih2, k) == (odt, oc, oi, ok)
+    # default dt is the reference's 0.5f*h/(c*sqrtf(3.f))
+    from virs_amd.fdtd import FDTD
+    g = FDTD((8, 8, 8), 343.0 / 88200.0)
+    assert g.constants == O.reference_constants(343.0 / 88200.0)
+    g.close()
+    f.close()
+
+
+def test_graph_replay_matches_direct(gpu):
+    p = tiny_problem(3, steps=95)
+    a = gpu_context(p, use_graph=False)
+    b = gpu_context(p, use_graph=True)
+    a.step(p["steps"])
+    # uneven batches: graph replays of 30 steps plus single launches
+    b.step(31)
+    b.step(64)
+    a.synchronize()
+    b.synchronize()
+    np.testing.assert_array_equal(a.receivers(True), b.receivers(True))
+    np.testing.assert_array_equal(a.field(0), b.field(0))
+    a.close()
+    b.close()
+
+
+def test_repeatable_and_zchunks_invariant(gpu):
+    """Jacobi-style update: two runs, and two different launch shapes, give
+    bitwise identical results (the race check)."""
+    p = tiny_problem(4, dims=(32, 20, 40), steps=80)
+    outs = []
+    for zc in (0, 1, 7, 40):
+        f = gpu_context(p, zchunks=zc)
+        f.step(p["steps"])
+        outs.append((f.receivers(True), f.field(0)))
+        f.close()
+    for ir, fld in outs[1:]:
+        np.testing.assert_array_equal(ir, outs[0][0])
+        np.testing.assert_array_equal(fld, outs[0][1])
+
+
+@pytest.mark.parametrize("nx", [23, 26, 30])
+def test_odd_widths(gpu, nx):
+    """Row widths that force 1- and 2-wide vectors along x."""
+    p = tiny_problem(5, dims=(nx, 15, 13), steps=40)
+    for prec in ("fp32", "fp64"):
+        f = gpu_context(p, prec)
+        f.step(p["steps"])
+        ref = oracle_problem(p).run(p["steps"], "f32" if prec == "fp32" else "f64", want_fields=True)
+        np.testing.assert_array_equal(f.receivers(True), ref["ir"])
+        np.testing.assert_array_equal(f.field(0), ref["p"])
+        f.close()
+
+
+def test_reference_hard_source_pulse(gpu):
+    """Flag-3 cells hold 10 for steps 0-9 then 0 (simulator_gpu.cu:214-218),
+    and a receiver on one reads exactly that."""
+    p = tiny_problem(6, steps=20)
+    src = int(np.flatnonzero(p["flags"].reshape(-1) == 3)[0])
+    p["rcv_idx"] = np.array([src], np.int64)
+    f = gpu_context(p)
+    f.set_hard_source(None)
+    f.step(20)
+    ir = f.receivers(True)[0]
+    np.testing.assert_array_equal(ir, np.where(np.arange(20) < 10, 10.0, 0.0))
+    f.close()
+
+
+def test_hard_source_signal(gpu):
+    p = tiny_problem(7, steps=30)
+    sig = np.sin(np.arange(30) * 0.3)
+    f = gpu_context(p)
+    f.set_hard_source(sig)
+    f.step(30)
+    pb = oracle_problem(p)
+    pb.hard_sig = sig
+    ref = pb.run(30, "f32", want_fields=True)
+    np.testing.assert_array_equal(f.receivers(True), ref["ir"])
+    np.testing.assert_array_equal(f.field(0), ref["p"])
+    f.close()
+
+
+def test_shoebox_2000_steps_vs_reference_scheme(gpu):
+    """The 5x4x3 m shoebox (160x132x108, 2000 steps): FP32 IR within 1e-4
+    relative L2 of the reference-literal oracle (FP32 filter state), and
+    bit-exact against the FP64-state FP32 oracle."""
+    from virs_amd import rooms
+    from virs_amd.fdtd import setup
+    prep = rooms.prepare(rooms.shoebox_5x4x3())
+    _say(f"shoebox nb={len(prep.room.bidx)}")
+    f = setup(prep, "fp32")
+    f.step(prep.scenario.steps)
+    ir = f.receivers(True)
+    f.close()
+    r = prep.room
+    O.set_threads(16)
+    kw = dict(bidx=r.bidx, bmat=r.bmat, coeffs=prep.scenario.coeffs, soft_idx=prep.src_idx,
+              soft_sig=prep.src_sig, rcv_idx=prep.rcv_idx, sponge_width=16, sponge_sigma=0.5)
+    lit = O.Problem(r.flags, prep.scenario.h, dt=prep.dt, state_f32=True, **kw).run(2000, "f32")["ir"]
+    same = O.Problem(r.flags, prep.scenario.h, dt=prep.dt, **kw).run(2000, "f32")["ir"]
+    err = rel_l2(ir, lit)
+    _say(f"shoebox fp32 vs reference-literal rel L2 = {err:.3e}")
+    assert err <= 1e-4
+    np.testing.assert_array_equal(ir, same)

@@ -1,0 +1,268 @@
+"""Pins the CPU oracle (oracle/fdtd_oracle.c) against what the mathematics
+fixes, since the reference holds no golden vectors or tests for this path.
+
+Each check is chosen so a plausible slip in the restatement fails it:
+  eigenmodes      wrong wall-mirror rule, wrong Courant constant, wrong sign
+  energy          asymmetric neighbour handling
+  reciprocity     a boundary correction that couples cells
+  lfilter         DF1 recursion order / coefficient order / sign convention
+  single cell     the 0.1/0.9/0.4/K correction constants
+  numpy float32   operand order and indexing of the FP32 dense update
+  finite diffs    the hand-written adjoint
+  RMS rule        the early-stop thresholds and smoothing
+"""
+import numpy as np
+import pytest
+from scipy.signal import lfilter
+
+from helpers import oracle_problem, rel_l2, tiny_problem
+from oracle import oracle as O
+
+
+def walled_box(nx, ny, nz):
+    f = np.zeros((nz, ny, nx), np.uint8)
+    f[0, :, :] = f[-1, :, :] = 1
+    f[:, 0, :] = f[:, -1, :] = 1
+    f[:, :, 0] = f[:, :, -1] = 1
+    return f
+
+
+@pytest.mark.parametrize("modes", [(1, 0, 0), (2, 1, 0), (1, 2, 3), (5, 4, 2)])
+def test_rigid_box_eigenmode(modes):
+    nx, ny, nz = 14, 11, 9
+    Lx, Ly, Lz = nx - 2, ny - 2, nz - 2
+    kx, ky, kz = modes
+    lam2 = 1.0 / 3.0
+    h = 0.05
+    dt = O.dt_for_lambda2(lam2, h)
+    pb0 = O.Problem(walled_box(nx, ny, nz), h, dt=dt)
+    lam2_f = float(pb0.c2dt2) * float(pb0.ih2)
+    ii = np.arange(nx) - 1 + 0.5
+    jj = np.arange(ny) - 1 + 0.5
+    kk = np.arange(nz) - 1 + 0.5
+    phi = (np.cos(np.pi * kz * kk / Lz)[:, None, None] * np.cos(np.pi * ky * jj / Ly)[None, :, None]
+           * np.cos(np.pi * kx * ii / Lx)[None, None, :])
+    phi[0, :, :] = phi[-1, :, :] = 0
+    phi[:, 0, :] = phi[:, -1, :] = 0
+    phi[:, :, 0] = phi[:, :, -1] = 0
+    mu = 4 * sum(np.sin(np.pi * k / (2 * L)) ** 2 for k, L in ((kx, Lx), (ky, Ly), (kz, Lz)))
+    w = np.arccos(1 - lam2_f * mu / 2)
+    rc = (4, 5, 6)
+    ridx = (rc[2] * ny + rc[1]) * nx + rc[0]
+    pb = O.Problem(walled_box(nx, ny, nz), h, dt=dt, init_p=phi.reshape(-1),
+                   init_q=(np.cos(w) * phi).reshape(-1), rcv_idx=[ridx])
+    n = 200
+    out = pb.run(n, "f64", want_fields=True)
+    np.testing.assert_allclose(out["p"], np.cos(w * n) * phi, atol=1e-10)
+    np.testing.assert_allclose(out["ir"][0], np.cos(w * np.arange(1, n + 1)) * phi.reshape(-1)[ridx],
+                               atol=1e-10)
+
+
+def _edge_energy(u1, u0, flags, lam2):
+    """||u1-u0||^2 + lam2 * sum over air-air face pairs of (du1)(du0)."""
+    air = flags != 1
+    e = np.sum((u1 - u0)[air] ** 2)
+    for ax in range(3):
+        a = np.swapaxes(air, 0, ax)
+        a1 = np.swapaxes(u1, 0, ax)
+        a0 = np.swapaxes(u0, 0, ax)
+        pair = a[1:] & a[:-1]
+        e += lam2 * np.sum(((a1[1:] - a1[:-1]) * (a0[1:] - a0[:-1]))[pair])
+    return e
+
+
+def test_energy_conserved_with_walls():
+    rng = np.random.default_rng(3)
+    nx, ny, nz = 16, 13, 11
+    f = walled_box(nx, ny, nz)
+    f[1:-1, 1:-1, 1:-1][rng.random((nz - 2, ny - 2, nx - 2)) < 0.08] = 1
+    p0 = rng.standard_normal(f.shape) * (f != 1)
+    q0 = rng.standard_normal(f.shape) * (f != 1)
+    h, lam2 = 0.05, 0.3
+    pb = O.Problem(f, h, dt=O.dt_for_lambda2(lam2, h), init_p=p0.reshape(-1), init_q=q0.reshape(-1))
+    l2 = float(pb.c2dt2) * float(pb.ih2)
+    e0 = _edge_energy(p0, q0, f, l2)
+    for n in (1, 17, 150):
+        out = pb.run(n, "f64", want_fields=True)
+        e = _edge_energy(out["p"], out["q"], f, l2)
+        assert abs(e - e0) <= 1e-10 * abs(e0)
+
+
+def test_reciprocity_with_boundary_filters():
+    p = tiny_problem(11, sponge=0, hard=False, steps=150)
+    a, b = int(p["soft_idx"][0]), int(p["rcv_idx"][0])
+    sig = p["soft_sig"][:1]
+    ab = O.Problem(p["flags"], p["h"], dt=p["dt"], bidx=p["bidx"], bmat=p["bmat"], coeffs=p["coeffs"],
+                   soft_idx=[a], soft_sig=sig, rcv_idx=[b]).run(150, "f64")["ir"][0]
+    ba = O.Problem(p["flags"], p["h"], dt=p["dt"], bidx=p["bidx"], bmat=p["bmat"], coeffs=p["coeffs"],
+                   soft_idx=[b], soft_sig=sig, rcv_idx=[a]).run(150, "f64")["ir"][0]
+    assert np.max(np.abs(ab)) > 1e-6
+    np.testing.assert_allclose(ab, ba, rtol=0, atol=1e-12 * np.max(np.abs(ab)))
+
+
+@pytest.mark.parametrize("state_f32", [False, True])
+def test_df1_matches_lfilter(state_f32):
+    p = tiny_problem(12, steps=80)
+    out = oracle_problem(p, state_f32=state_f32).run(80, "f64" if not state_f32 else "f32",
+                                                   want_trace=True)
+    td, ty = out["trace_d"], out["trace_y"]
+    tol = 1e-12 if not state_f32 else 2e-5
+    for e in range(0, len(p["bidx"]), 37):
+        c = p["coeffs"][p["bmat"][e]]
+        if state_f32:
+            c = c.astype(np.float32).astype(np.float64)
+        y = lfilter(c[:3], [1.0, c[3], c[4]], td[:, e])
+        scale = max(np.max(np.abs(y)), 1e-30)
+        assert np.max(np.abs(y - ty[:, e])) <= tol * scale
+
+
+def test_single_cell_boundary_update():
+    """One interior cell with one wall face neighbour and five pressure-release
+    neighbours, hand-derived from simulator_gpu.cu:83-88 and :130-148:
+      R = 2 pc - q + c2dt2 ((pc + 0*5 - 6 pc) ih2)
+      R -= K (0.1 y + 0.9 d) 0.4f,  d = pc - q,  y = b0 d (zero state)."""
+    f = np.zeros((3, 3, 3), np.uint8)
+    f[1, 1, 2] = 1  # +x neighbour is a wall
+    f[1, 1, 1] = 2
+    ci = 13
+    pc, q = 0.8125, 0.3
+    c = np.array([[0.6, -0.2, 0.05, -0.3, 0.1]])
+    p0 = np.zeros(27)
+    q0 = np.zeros(27)
+    p0[ci], q0[ci] = pc, q
+    h = 0.04
+    pb = O.Problem(f, h, dt=O.dt_for_lambda2(1 / 3, h), bidx=[ci], coeffs=c, rcv_idx=[ci],
+                   init_p=p0, init_q=q0)
+    got = pb.run(1, "f64")["ir"][0, 0]
+    c2dt2, ih2, K = float(pb.c2dt2), float(pb.ih2), float(pb.kconst)
+    dense = 2 * pc - q + c2dt2 * ((pc - 6 * pc) * ih2)
+    d = pc - q
+    y = 0.6 * d
+    want = dense - K * (0.1 * y + 0.9 * d) * float(np.float32(0.4))
+    assert got == pytest.approx(want, rel=1e-15, abs=0)
+
+
+def _numpy_dense_f32(P, Q, flags, c2dt2, ih2):
+    """Vectorised float32 restatement of the reference dense update."""
+    R = np.zeros_like(P)
+    f = flags
+    pc = P[1:-1, 1:-1, 1:-1]
+
+    def V(sl):
+        return np.where(f[sl] == 1, pc, P[sl])
+
+    s = (V((slice(1, -1), slice(1, -1), slice(2, None))) + V((slice(1, -1), slice(1, -1), slice(0, -2))))
+    s = s + V((slice(1, -1), slice(2, None), slice(1, -1)))
+    s = s + V((slice(1, -1), slice(0, -2), slice(1, -1)))
+    s = s + V((slice(2, None), slice(1, -1), slice(1, -1)))
+    s = s + V((slice(0, -2), slice(1, -1), slice(1, -1)))
+    lap = (s - np.float32(6) * pc) * ih2
+    upd = (np.float32(2) * pc - np.float32(1) * Q[1:-1, 1:-1, 1:-1]) + c2dt2 * lap
+    fi = f[1:-1, 1:-1, 1:-1]
+    R[1:-1, 1:-1, 1:-1] = np.where(fi == 1, np.float32(0), np.where((fi == 0) | (fi == 2), upd, np.float32(0)))
+    return R
+
+
+def test_float32_dense_matches_numpy():
+    rng = np.random.default_rng(5)
+    nx, ny, nz = 17, 12, 10
+    f = np.zeros((nz, ny, nx), np.uint8)
+    f[rng.random(f.shape) < 0.15] = 1
+    f[rng.random(f.shape) < 0.1] = 2
+    P = (rng.standard_normal(f.shape)).astype(np.float32)
+    Q = (rng.standard_normal(f.shape)).astype(np.float32)
+    for arr in (P, Q):
+        arr[0], arr[-1], arr[:, 0], arr[:, -1], arr[:, :, 0], arr[:, :, -1] = 0, 0, 0, 0, 0, 0
+    h = 0.04
+    pb = O.Problem(f, h, dt=O.dt_for_lambda2(0.3, h), init_p=P.reshape(-1).astype(np.float64),
+                   init_q=Q.reshape(-1).astype(np.float64))
+    P1, Q1 = P, Q
+    for n in (1, 2, 3):
+        R = _numpy_dense_f32(P1, Q1, f, pb.c2dt2, pb.ih2)
+        P1, Q1 = R, P1
+        out = pb.run(n, "f32", want_fields=True)
+        np.testing.assert_array_equal(out["p"], P1.astype(np.float64))
+
+
+def test_adjoint_matches_central_differences():
+    p = tiny_problem(21, dims=(14, 12, 10), steps=70, sponge=2)
+    rng = np.random.default_rng(0)
+    pb = oracle_problem(p)
+    n = p["steps"]
+    target = rng.standard_normal((len(p["rcv_idx"]), n)) * 1e-3
+
+    def loss(coeffs):
+        pb.coeffs = np.ascontiguousarray(coeffs)
+        ir = pb.run(n, "f64")["ir"]
+        return float(np.sum((ir - target) ** 2)), ir
+
+    c0 = p["coeffs"].copy()
+    L0, ir0 = loss(c0)
+    pb.coeffs = np.ascontiguousarray(c0)
+    g = pb.adjoint(n, 2 * (ir0 - target))
+    hstep = 1e-3
+    fd = np.zeros_like(c0)
+    for m in range(c0.shape[0]):
+        for k in range(5):
+            cp = c0.copy()
+            cm = c0.copy()
+            cp[m, k] += hstep
+            cm[m, k] -= hstep
+            fd[m, k] = (loss(cp)[0] - loss(cm)[0]) / (2 * hstep)
+    assert rel_l2(g, fd) < 1e-5
+
+
+def test_rms_rule_closed_forms():
+    # nothing can stop at or before step 3000
+    assert O.rms_continue_sequence(np.zeros(3001)) == -1
+    # silence in the 100-sample window: rms 0 < target at the first check
+    ir = np.ones(3200, np.float32)
+    ir[2900:] = 0
+    assert O.rms_continue_sequence(ir) == 3001
+    # a constant amplitude above the target never stops
+    assert O.rms_continue_sequence(np.full(6000, 0.002, np.float32)) == -1
+    # just below the target stops at the first check
+    assert O.rms_continue_sequence(np.full(3100, 0.0008, np.float32)) == 3001
+
+
+def test_sponge_single_cell_closed_form():
+    """A lone interior cell walled on all six faces (L = 0) at edge distance 1
+    of a 2-cell sponge: p^{n+1} = a p^n - b p^{n-1} with roots 1 and
+    b = (1-s)/(1+s), s = sigma/8: p^n = A + B b^n."""
+    f = np.ones((3, 3, 3), np.uint8)
+    f[1, 1, 1] = 0
+    p0 = np.zeros(27)
+    p0[13] = 1.0
+    sigma = 0.75
+    pb = O.Problem(f, 0.04, dt=O.dt_for_lambda2(1 / 3, 0.04), init_p=p0, rcv_idx=[13],
+                   sponge_width=2, sponge_sigma=sigma)
+    a, b = float(pb.spg[0][1]), float(pb.spg[1][1])
+    s = 0.5 * sigma * 0.25
+    assert a == pytest.approx(2 / (1 + s), rel=1e-7)
+    assert b == pytest.approx((1 - s) / (1 + s), rel=1e-7)
+    ir = pb.run(40, "f64")["ir"][0]
+    # p^0 = 1, p^-1 = 0: A + B = 1, A + B/b = 0
+    B = b / (b - 1)
+    A = 1 - B
+    n = np.arange(1, 41)
+    np.testing.assert_allclose(ir, A + B * b ** n, rtol=1e-12)
+
+
+def test_reference_cpu_step_single_cell():
+    """simulator_cpu.cpp:55-81 on one cell with pressure-release neighbours:
+    p^{n+1} = (2 - g dt) p - (1 - g dt) q + c2dt2 (-6 p) ih2 in float32."""
+    f = np.zeros((3, 3, 3), np.uint8)
+    p0 = np.zeros((3, 3, 3), np.float32)
+    p0[1, 1, 1] = 1.0
+    h = np.float32(0.01)
+    out = O.refcpu_steps(f, float(h), 3, p=p0)
+    c = np.float32(343)
+    dt = np.float32(0.5 * float(h) / float(c * np.sqrt(np.float32(3))))
+    c2 = c * c * dt * dt
+    ih2 = np.float32(1) / (h * h)
+    damp = np.float32(5) * dt
+    p, q = np.float32(1), np.float32(0)
+    for _ in range(3):
+        lap = (np.float32(0) - np.float32(6) * p) * ih2
+        p, q = (np.float32(2) - damp) * p - (np.float32(1) - damp) * q + c2 * lap, p
+    assert out[1, 1, 1] == p

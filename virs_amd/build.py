@@ -1,0 +1,61 @@
+"""Build recipe for the native libraries (in-tree, gfx950 only).
+
+    python -m virs_amd.build
+
+produces
+    virs_amd/lib/libvirs_hip.so   HIP kernels + C ABI   (include/virs_fdtd.h)
+    virs_amd/lib/libvirs_host.so  host grid pipeline     (include/virs_host.h)
+"""
+import os
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "virs_amd")
+LIB = os.path.join(PKG, "lib")
+INC = os.path.join(ROOT, "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off",
+    # packed FP32 instructions stay off (see the kernels' header)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    f"-I{INC}",
+]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-pthread", f"-I{INC}"]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"build failed: {' '.join(cmd)}")
+
+
+def build(verbose=False):
+    os.makedirs(LIB, exist_ok=True)
+    hip_src = os.path.join(PKG, "csrc", "fdtd", "virs_fdtd.hip")
+    host_src = os.path.join(PKG, "csrc", "host", "virs_host.cpp")
+    hip_out = os.path.join(LIB, "libvirs_hip.so")
+    host_out = os.path.join(LIB, "libvirs_host.so")
+    _run(["g++"] + HOST_FLAGS + [host_src, "-o", host_out + ".tmp"], verbose)
+    os.replace(host_out + ".tmp", host_out)
+    _run([HIPCC] + HIP_FLAGS + [hip_src, "-o", hip_out + ".tmp"], verbose)
+    os.replace(hip_out + ".tmp", hip_out)
+    return hip_out, host_out
+
+
+def build_oracle(verbose=False):
+    cmd = ["make", "-s", "-C", os.path.join(ROOT, "oracle")]
+    _run(cmd, verbose)
+    return os.path.join(ROOT, "oracle", "_build", "libvirs_oracle.so")
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    build_oracle(verbose=True)

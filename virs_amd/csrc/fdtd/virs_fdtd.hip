@@ -1,0 +1,1122 @@
+// libvirs_hip.so: the C ABI of include/virs_fdtd.h on top of kernels.hpp.
+//
+// Host orchestration of the step loop (the reference's fdtd_gpu_step,
+// src/simulator/simulator_gpu.cu:184-332), without its per-step
+// cudaMemcpyToSymbol, three device synchronisations and 4-byte receiver
+// readback: a step is two stream-ordered launches (stencil, then boundary +
+// points + step counter), receivers accumulate in a device buffer, and
+// batches of steps can be replayed as HIP graphs.
+#include "kernels.hpp"
+#include "virs_fdtd.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace virs;
+
+struct virs_ctx {
+    virs_desc d{};
+    hipStream_t stream = nullptr;
+    long long nx = 0, ny = 0, nz = 0, N = 0;
+    int vw = 1;            // cells per thread along x in the stencil
+    size_t esz = 4;        // bytes per stored pressure
+    float dt = 0, c2dt2 = 0, ih2 = 0, kconst = 0;
+    Sponge sp{};
+
+    void *f[3] = {nullptr, nullptr, nullptr};
+    int ip = 0, iq = 1, ir = 2; // rotation: P, Q, R
+    uint8_t *code = nullptr;
+    uint8_t *normals = nullptr;
+    bool have_flags = false;
+
+    long long nb = 0;
+    uint32_t *bidx = nullptr;
+    uint8_t *bmat = nullptr;
+    double *st[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<uint32_t> h_bidx;
+    std::vector<uint8_t> h_bmat;
+
+    double *coeffs = nullptr;
+    int nmat = 0;
+
+    StepCtl *ctl = nullptr;
+    StepCtl *ctl_adj = nullptr;
+
+    double *hard_sig = nullptr;
+    long long hard_len = 0;
+    int hard_ref = 1;
+
+    int nsoft = 0;
+    long long soft_len = 0;
+    long long *soft_idx = nullptr, *soft_bpos = nullptr;
+    double *soft_sig = nullptr;
+    std::vector<long long> h_soft_idx;
+    std::vector<double> h_soft_sig;
+
+    int nrcv = 0;
+    long long *rcv_idx = nullptr, *rcv_bpos = nullptr;
+    std::vector<long long> h_rcv_idx;
+    double *ir_buf = nullptr;
+    long long ir_cap = 0;
+
+    long long steps = 0;   // steps run since virs_start_simulation
+    long long counted = 0; // steps reported (early stop truncates)
+    bool stopped = false;
+    int es_enable = 0;
+    float es_target = 0.0009f, rms_state = 0.f, percent = 0.f;
+    std::vector<float> ir0_host;
+
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    int graph_steps = 0;
+
+    // adjoint
+    int ncp = 0;
+    long long rec_steps = 0, seg = 0;
+    std::vector<void *> cp_field;   // [ncp][2] stored fields
+    std::vector<double *> cp_state; // [ncp] 4*nb
+    double *trace_d = nullptr, *trace_y = nullptr;
+    long long trace_slots = 0;
+    long long last_seg_base = -1;   // segment whose trace is already on the device
+    void *af[3] = {nullptr, nullptr, nullptr};
+    double *ast[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *gsig = nullptr;
+    double *grad_part = nullptr;
+    long long grad_blocks = 0;
+
+    std::string err;
+};
+
+namespace {
+
+int fail(virs_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPC(ctx, expr)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail((ctx), VIRS_ERR_HIP, "%s failed: %s (%s:%d)", #expr,                  \
+                        hipGetErrorString(e_), __FILE__, __LINE__);                            \
+    } while (0)
+
+template <typename P>
+int dalloc(virs_ctx *c, P **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(c, VIRS_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    return VIRS_OK;
+}
+
+template <typename P>
+void dfree(P *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+int grid_for(long long n) { return (int)std::min<long long>(std::max<long long>((n + 255) / 256, 1), 65536); }
+
+void drop_graphs(virs_ctx *c)
+{
+    for (auto &g : c->graph)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+// Position of linear index `idx` in the sorted boundary list, or -1.
+long long bpos_of(const virs_ctx *c, long long idx)
+{
+    auto it = std::lower_bound(c->h_bidx.begin(), c->h_bidx.end(), (uint32_t)idx);
+    if (idx <= (long long)UINT32_MAX && it != c->h_bidx.end() && (long long)*it == idx)
+        return (long long)(it - c->h_bidx.begin());
+    return -1;
+}
+
+int upload_points(virs_ctx *c)
+{
+    std::vector<long long> sb(c->nsoft), rb(c->nrcv);
+    for (int k = 0; k < c->nsoft; ++k) sb[k] = bpos_of(c, c->h_soft_idx[k]);
+    for (int k = 0; k < c->nrcv; ++k) rb[k] = bpos_of(c, c->h_rcv_idx[k]);
+    if (c->nsoft) {
+        HIPC(c, hipMemcpy(c->soft_idx, c->h_soft_idx.data(), sizeof(long long) * c->nsoft, hipMemcpyHostToDevice));
+        HIPC(c, hipMemcpy(c->soft_bpos, sb.data(), sizeof(long long) * c->nsoft, hipMemcpyHostToDevice));
+    }
+    if (c->nrcv) {
+        HIPC(c, hipMemcpy(c->rcv_idx, c->h_rcv_idx.data(), sizeof(long long) * c->nrcv, hipMemcpyHostToDevice));
+        HIPC(c, hipMemcpy(c->rcv_bpos, rb.data(), sizeof(long long) * c->nrcv, hipMemcpyHostToDevice));
+    }
+    return VIRS_OK;
+}
+
+DenseParams dense_params(const virs_ctx *c, const void *P, const void *Q, void *R, StepCtl *ctl,
+                         int &grid)
+{
+    DenseParams p{};
+    p.P = P;
+    p.Q = Q;
+    p.R = R;
+    p.code = c->code;
+    p.nx = (int)c->nx;
+    p.ny = (int)c->ny;
+    p.nz = (int)c->nz;
+    p.sxy = c->nx * c->ny;
+    p.nvec_plane = p.sxy / c->vw;
+    p.nchunks_plane = (int)((p.nvec_plane + kBlock - 1) / kBlock);
+    int zch = c->d.zchunks;
+    if (zch <= 0) {
+        // enough blocks to fill 256 CUs several times over, rounded to a
+        // multiple of 8 so every XCD gets an equal contiguous share
+        const long long want = 4096;
+        zch = (int)std::max<long long>(1, (want + p.nchunks_plane - 1) / p.nchunks_plane);
+    }
+    zch = (int)std::min<long long>(zch, c->nz);
+    p.zc = (int)((c->nz + zch - 1) / zch);
+    const int nzc = (int)((c->nz + p.zc - 1) / p.zc);
+    grid = p.nchunks_plane * nzc;
+    p.c2dt2 = c->c2dt2;
+    p.ih2 = c->ih2;
+    p.ctl = ctl;
+    p.hard_sig = c->hard_sig;
+    p.hard_len = c->hard_len;
+    p.hard_ref = c->hard_ref;
+    p.nsoft = c->nsoft;
+    p.soft_sig = c->soft_sig;
+    p.soft_len = c->soft_len;
+    p.sp = c->sp;
+    return p;
+}
+
+TailParams tail_params(const virs_ctx *c, const void *P, const void *Q, void *R, StepCtl *ctl)
+{
+    TailParams t{};
+    t.P = P;
+    t.Q = Q;
+    t.R = R;
+    t.nb = c->nb;
+    t.bidx = c->bidx;
+    t.bmat = c->bmat;
+    t.s0 = c->st[0];
+    t.s1 = c->st[1];
+    t.s2 = c->st[2];
+    t.s3 = c->st[3];
+    t.coeffs = c->coeffs;
+    t.nmat = c->nmat;
+    t.kconst = c->kconst;
+    t.ctl = ctl;
+    t.nsoft = c->nsoft;
+    t.soft_idx = c->soft_idx;
+    t.soft_bpos = c->soft_bpos;
+    t.soft_sig = c->soft_sig;
+    t.soft_len = c->soft_len;
+    t.nrcv = c->nrcv;
+    t.rcv_idx = c->rcv_idx;
+    t.rcv_bpos = c->rcv_bpos;
+    t.ir = c->ir_buf;
+    t.ir_cap = c->ir_cap;
+    t.nx = (int)c->nx;
+    t.ny = (int)c->ny;
+    t.nz = (int)c->nz;
+    t.sp = c->sp;
+    return t;
+}
+
+int tail_grid(const virs_ctx *c) { return (int)std::max<long long>(1, (c->nb + kBlock - 1) / kBlock); }
+
+template <typename T, bool ADJ>
+hipError_t launch_dense_t(const virs_ctx *c, const DenseParams &p, int grid)
+{
+    switch (c->vw) {
+    case 8: hipLaunchKernelGGL((k_dense<T, 8, ADJ>), dim3(grid), dim3(kBlock), 0, c->stream, p); break;
+    case 4: hipLaunchKernelGGL((k_dense<T, 4, ADJ>), dim3(grid), dim3(kBlock), 0, c->stream, p); break;
+    case 2: hipLaunchKernelGGL((k_dense<T, 2, ADJ>), dim3(grid), dim3(kBlock), 0, c->stream, p); break;
+    default: hipLaunchKernelGGL((k_dense<T, 1, ADJ>), dim3(grid), dim3(kBlock), 0, c->stream, p); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_dense(const virs_ctx *c, const DenseParams &p, int grid, bool adj)
+{
+    switch (c->d.precision) {
+    case VIRS_FP16: return launch_dense_t<f16, false>(c, p, grid);
+    case VIRS_BF16: return launch_dense_t<bf16, false>(c, p, grid);
+    case VIRS_FP64: return adj ? launch_dense_t<double, true>(c, p, grid) : launch_dense_t<double, false>(c, p, grid);
+    default: return adj ? launch_dense_t<float, true>(c, p, grid) : launch_dense_t<float, false>(c, p, grid);
+    }
+}
+
+hipError_t launch_tail(const virs_ctx *c, const TailParams &t)
+{
+    const int g = tail_grid(c);
+    switch (c->d.precision) {
+    case VIRS_FP16: hipLaunchKernelGGL(k_tail<f16>, dim3(g), dim3(kBlock), 0, c->stream, t); break;
+    case VIRS_BF16: hipLaunchKernelGGL(k_tail<bf16>, dim3(g), dim3(kBlock), 0, c->stream, t); break;
+    case VIRS_FP64: hipLaunchKernelGGL(k_tail<double>, dim3(g), dim3(kBlock), 0, c->stream, t); break;
+    default: hipLaunchKernelGGL(k_tail<float>, dim3(g), dim3(kBlock), 0, c->stream, t); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_tail_adj(const virs_ctx *c, const TailParams &t)
+{
+    const int g = tail_grid(c);
+    if (c->d.precision == VIRS_FP64)
+        hipLaunchKernelGGL(k_tail_adj<double>, dim3(g), dim3(kBlock), 0, c->stream, t);
+    else
+        hipLaunchKernelGGL(k_tail_adj<float>, dim3(g), dim3(kBlock), 0, c->stream, t);
+    return hipGetLastError();
+}
+
+// One forward step on the context's current rotation; optionally records the
+// boundary trace for the adjoint.
+int launch_step(virs_ctx *c, double *trace_d = nullptr, double *trace_y = nullptr,
+                long long trace_base = 0, bool receivers = true, hipEvent_t *ev = nullptr)
+{
+    int grid = 0;
+    DenseParams p = dense_params(c, c->f[c->ip], c->f[c->iq], c->f[c->ir], c->ctl, grid);
+    if (ev) HIPC(c, hipEventRecord(ev[0], c->stream));
+    HIPC(c, launch_dense(c, p, grid, false));
+    if (ev) HIPC(c, hipEventRecord(ev[1], c->stream));
+    TailParams t = tail_params(c, c->f[c->ip], c->f[c->iq], c->f[c->ir], c->ctl);
+    if (!receivers) t.nrcv = 0;
+    t.trace_d = trace_d;
+    t.trace_y = trace_y;
+    t.trace_base = trace_base;
+    t.trace_slots = c->trace_slots;
+    HIPC(c, launch_tail(c, t));
+    if (ev) HIPC(c, hipEventRecord(ev[2], c->stream));
+    const int ip = c->ip;
+    c->ip = c->ir;
+    c->ir = c->iq;
+    c->iq = ip;
+    return VIRS_OK;
+}
+
+int check_ready(virs_ctx *c)
+{
+    if (!c) return VIRS_ERR_ARG;
+    if (!c->have_flags) return fail(c, VIRS_ERR_STATE, "flags not uploaded");
+    if (!c->coeffs && c->nb > 0) return fail(c, VIRS_ERR_STATE, "materials not set");
+    if (c->ir_cap == 0 && c->nrcv > 0) return fail(c, VIRS_ERR_STATE, "virs_start_simulation not called");
+    HIPC(c, hipSetDevice(c->d.device));
+    return VIRS_OK;
+}
+
+size_t elem_size(int prec)
+{
+    switch (prec) {
+    case VIRS_FP16:
+    case VIRS_BF16: return 2;
+    case VIRS_FP64: return 8;
+    default: return 4;
+    }
+}
+
+int zero_fields(virs_ctx *c, void *const *f)
+{
+    for (int k = 0; k < 3; ++k)
+        if (f[k]) HIPC(c, hipMemsetAsync(f[k], 0, c->esz * (size_t)c->N, c->stream));
+    return VIRS_OK;
+}
+
+int reset_run(virs_ctx *c)
+{
+    int rc = zero_fields(c, c->f);
+    if (rc) return rc;
+    for (int k = 0; k < 4; ++k)
+        if (c->st[k] && c->nb) HIPC(c, hipMemsetAsync(c->st[k], 0, sizeof(double) * c->nb, c->stream));
+    hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl, 0LL, 1);
+    HIPC(c, hipGetLastError());
+    c->ip = 0;
+    c->iq = 1;
+    c->ir = 2;
+    c->steps = c->counted = 0;
+    c->stopped = false;
+    c->rms_state = c->percent = 0.f;
+    c->ir0_host.clear();
+    drop_graphs(c);
+    return VIRS_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int virs_desc_default(virs_desc *d)
+{
+    if (!d) return VIRS_ERR_ARG;
+    std::memset(d, 0, sizeof *d);
+    d->h = 343.f / (22050.f * 2.f * (float)2);
+    d->c = 343.f;
+    d->dt = 0.f;
+    d->precision = VIRS_FP32;
+    d->sponge_width = 0;
+    d->sponge_sigma = 0.f;
+    d->device = 0;
+    d->use_graph = 0;
+    d->zchunks = 0;
+    return VIRS_OK;
+}
+
+int virs_create(const virs_desc *d, virs_ctx **out)
+{
+    if (!d || !out) return VIRS_ERR_ARG;
+    *out = nullptr;
+    auto *c = new virs_ctx();
+    c->d = *d;
+    c->nx = d->nx;
+    c->ny = d->ny;
+    c->nz = d->nz;
+    if (c->nx < 3 || c->ny < 3 || c->nz < 3 || c->nx > (1 << 30) || c->ny > (1 << 30) ||
+        c->nz > (1 << 30) || !(d->h > 0.f) || !(d->c > 0.f) || d->precision < 0 ||
+        d->precision > 3 || d->sponge_width < 0 || d->sponge_width > VIRS_MAX_SPONGE) {
+        delete c;
+        return VIRS_ERR_ARG;
+    }
+    c->N = c->nx * c->ny * c->nz;
+    c->esz = elem_size(d->precision);
+    const int maxv = d->precision == VIRS_FP64 ? 2 : d->precision == VIRS_FP32 ? 4 : 8;
+    c->vw = 1;
+    for (int v = maxv; v > 1; v >>= 1)
+        if (c->nx % v == 0) {
+            c->vw = v;
+            break;
+        }
+    // Constants exactly as simulator_gpu.cu:195-198 and :147 compute them.
+    const float cs = d->c, h = d->h;
+    c->dt = d->dt > 0.f ? d->dt : 0.5f * h / (cs * std::sqrt(3.f));
+    c->c2dt2 = cs * cs * c->dt * c->dt;
+    c->ih2 = 1.f / (h * h);
+    c->kconst = (cs * c->dt) / h * 0.95f;
+    // Sponge profile sigma(d) = sigma_max ((W-d)/W)^2 with the centred
+    // damping update R = (2P - (1-s)Q + L)/(1+s), s = sigma/2, which stays
+    // stable up to the Courant limit.  Outside the layer a,b,g = 2,1,1.
+    c->sp.w = d->sponge_width;
+    for (int k = 0; k < d->sponge_width; ++k) {
+        const double t = (double)(d->sponge_width - k) / (double)d->sponge_width;
+        const double s = 0.5 * (double)d->sponge_sigma * t * t;
+        c->sp.a[k] = (float)(2.0 / (1.0 + s));
+        c->sp.b[k] = (float)((1.0 - s) / (1.0 + s));
+        c->sp.g[k] = (float)(1.0 / (1.0 + s));
+    }
+    hipError_t e = hipSetDevice(d->device);
+    if (e != hipSuccess) {
+        delete c;
+        return VIRS_ERR_HIP;
+    }
+    int rc = VIRS_OK;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = VIRS_ERR_HIP;
+    for (int k = 0; k < 3 && !rc; ++k) rc = dalloc(c, &c->f[k], c->esz * (size_t)c->N);
+    if (!rc) rc = dalloc(c, &c->code, (size_t)c->N);
+    if (!rc) rc = dalloc(c, &c->ctl, sizeof(StepCtl));
+    if (!rc) rc = dalloc(c, &c->ctl_adj, sizeof(StepCtl));
+    if (!rc) rc = dalloc(c, &c->soft_idx, sizeof(long long) * kMaxPoints);
+    if (!rc) rc = dalloc(c, &c->soft_bpos, sizeof(long long) * kMaxPoints);
+    if (!rc) rc = dalloc(c, &c->rcv_idx, sizeof(long long) * kMaxPoints);
+    if (!rc) rc = dalloc(c, &c->rcv_bpos, sizeof(long long) * kMaxPoints);
+    if (!rc) rc = reset_run(c);
+    if (!rc && hipMemsetAsync(c->code, kTypeZero << 6, (size_t)c->N, c->stream) != hipSuccess) rc = VIRS_ERR_HIP;
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = VIRS_ERR_HIP;
+    if (rc) {
+        virs_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return VIRS_OK;
+}
+
+int virs_destroy(virs_ctx *c)
+{
+    if (!c) return VIRS_ERR_ARG;
+    (void)hipSetDevice(c->d.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
+    for (auto &p : c->f) dfree(p);
+    for (auto &p : c->af) dfree(p);
+    for (auto &p : c->st) dfree(p);
+    for (auto &p : c->ast) dfree(p);
+    for (auto &p : c->cp_field) dfree(p);
+    for (auto &p : c->cp_state) dfree(p);
+    dfree(c->code);
+    dfree(c->normals);
+    dfree(c->bidx);
+    dfree(c->bmat);
+    dfree(c->coeffs);
+    dfree(c->ctl);
+    dfree(c->ctl_adj);
+    dfree(c->hard_sig);
+    dfree(c->soft_idx);
+    dfree(c->soft_bpos);
+    dfree(c->soft_sig);
+    dfree(c->rcv_idx);
+    dfree(c->rcv_bpos);
+    dfree(c->ir_buf);
+    dfree(c->trace_d);
+    dfree(c->trace_y);
+    dfree(c->gsig);
+    dfree(c->grad_part);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return VIRS_OK;
+}
+
+const char *virs_last_error(const virs_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int virs_get_constants(const virs_ctx *c, float *dt, float *c2dt2, float *ih2, float *kconst)
+{
+    if (!c) return VIRS_ERR_ARG;
+    if (dt) *dt = c->dt;
+    if (c2dt2) *c2dt2 = c->c2dt2;
+    if (ih2) *ih2 = c->ih2;
+    if (kconst) *kconst = c->kconst;
+    return VIRS_OK;
+}
+
+int virs_upload_flags(virs_ctx *c, const uint8_t *flags)
+{
+    if (!c || !flags) return VIRS_ERR_ARG;
+    for (long long i = 0; i < c->N; ++i)
+        if (flags[i] > 3) return fail(c, VIRS_ERR_ARG, "flag value %d at %lld (expected 0..3)", flags[i], i);
+    HIPC(c, hipSetDevice(c->d.device));
+    uint8_t *dflags = nullptr;
+    int rc = dalloc(c, &dflags, (size_t)c->N);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(dflags, flags, (size_t)c->N, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_build_code, dim3(4096), dim3(256), 0, c->stream, dflags, c->code,
+                           (int)c->nx, (int)c->ny, (int)c->nz);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dflags);
+    if (e != hipSuccess) return fail(c, VIRS_ERR_HIP, "flag upload: %s", hipGetErrorString(e));
+    c->have_flags = true;
+    drop_graphs(c);
+    return VIRS_OK;
+}
+
+int virs_upload_normals(virs_ctx *c, const uint8_t *normals)
+{
+    if (!c || !normals) return VIRS_ERR_ARG;
+    HIPC(c, hipSetDevice(c->d.device));
+    if (!c->normals) {
+        int rc = dalloc(c, &c->normals, (size_t)c->N);
+        if (rc) return rc;
+    }
+    HIPC(c, hipMemcpy(c->normals, normals, (size_t)c->N, hipMemcpyHostToDevice));
+    return VIRS_OK;
+}
+
+int virs_upload_boundary(virs_ctx *c, const uint32_t *idx, const uint8_t *mat, int64_t nb)
+{
+    if (!c || nb < 0 || (nb > 0 && !idx)) return VIRS_ERR_ARG;
+    for (int64_t e = 0; e < nb; ++e) {
+        if ((long long)idx[e] >= c->N) return fail(c, VIRS_ERR_ARG, "boundary index %u out of range", idx[e]);
+        if (e && idx[e] <= idx[e - 1]) return fail(c, VIRS_ERR_ARG, "boundary list not strictly ascending at %lld", (long long)e);
+        if (mat && c->nmat && mat[e] >= c->nmat) return fail(c, VIRS_ERR_ARG, "material %d >= %d", mat[e], c->nmat);
+    }
+    HIPC(c, hipSetDevice(c->d.device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    dfree(c->bidx);
+    dfree(c->bmat);
+    for (auto &p : c->st) dfree(p);
+    c->nb = nb;
+    c->h_bidx.assign(idx, idx + nb);
+    if (mat) c->h_bmat.assign(mat, mat + nb);
+    else c->h_bmat.clear();
+    int rc = dalloc(c, &c->bidx, sizeof(uint32_t) * nb);
+    for (int k = 0; k < 4 && !rc; ++k) rc = dalloc(c, &c->st[k], sizeof(double) * nb);
+    if (!rc && mat) rc = dalloc(c, &c->bmat, (size_t)nb);
+    if (rc) return rc;
+    if (nb) HIPC(c, hipMemcpy(c->bidx, idx, sizeof(uint32_t) * nb, hipMemcpyHostToDevice));
+    if (nb && mat) HIPC(c, hipMemcpy(c->bmat, mat, (size_t)nb, hipMemcpyHostToDevice));
+    for (int k = 0; k < 4; ++k)
+        if (nb) HIPC(c, hipMemset(c->st[k], 0, sizeof(double) * nb));
+    drop_graphs(c);
+    return upload_points(c);
+}
+
+int virs_set_materials(virs_ctx *c, const double *coeffs, int32_t nmat)
+{
+    if (!c || !coeffs || nmat <= 0 || nmat > 255) return VIRS_ERR_ARG;
+    for (size_t e = 0; e < c->h_bmat.size(); ++e)
+        if (c->h_bmat[e] >= nmat) return fail(c, VIRS_ERR_ARG, "boundary material %d >= %d", c->h_bmat[e], nmat);
+    HIPC(c, hipSetDevice(c->d.device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    dfree(c->coeffs);
+    int rc = dalloc(c, &c->coeffs, sizeof(double) * 5 * nmat);
+    if (rc) return rc;
+    HIPC(c, hipMemcpy(c->coeffs, coeffs, sizeof(double) * 5 * nmat, hipMemcpyHostToDevice));
+    c->nmat = nmat;
+    drop_graphs(c);
+    return VIRS_OK;
+}
+
+int virs_set_hard_source(virs_ctx *c, const double *sig, int64_t len)
+{
+    if (!c || len < 0 || (sig && len == 0)) return VIRS_ERR_ARG;
+    HIPC(c, hipSetDevice(c->d.device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    dfree(c->hard_sig);
+    c->hard_ref = sig ? 0 : 1;
+    c->hard_len = sig ? len : 0;
+    if (sig) {
+        int rc = dalloc(c, &c->hard_sig, sizeof(double) * len);
+        if (rc) return rc;
+        HIPC(c, hipMemcpy(c->hard_sig, sig, sizeof(double) * len, hipMemcpyHostToDevice));
+    }
+    drop_graphs(c);
+    return VIRS_OK;
+}
+
+int virs_set_soft_sources(virs_ctx *c, const int64_t *idx, const double *sig, int32_t nsrc, int64_t len)
+{
+    if (!c || nsrc < 0 || nsrc > kMaxPoints || len < 0 || (nsrc && (!idx || !sig || len == 0)))
+        return VIRS_ERR_ARG;
+    for (int k = 0; k < nsrc; ++k) {
+        if (idx[k] < 0 || idx[k] >= c->N) return fail(c, VIRS_ERR_ARG, "source %d index out of range", k);
+        for (int j = 0; j < k; ++j)
+            if (idx[j] == idx[k]) return fail(c, VIRS_ERR_ARG, "sources %d and %d share a cell", j, k);
+    }
+    HIPC(c, hipSetDevice(c->d.device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    dfree(c->soft_sig);
+    c->nsoft = nsrc;
+    c->soft_len = nsrc ? len : 0;
+    c->h_soft_idx.assign(idx, idx + nsrc);
+    c->h_soft_sig.assign(sig, sig + (size_t)nsrc * (size_t)c->soft_len);
+    if (nsrc) {
+        int rc = dalloc(c, &c->soft_sig, sizeof(double) * nsrc * len);
+        if (rc) return rc;
+        HIPC(c, hipMemcpy(c->soft_sig, sig, sizeof(double) * nsrc * len, hipMemcpyHostToDevice));
+    }
+    drop_graphs(c);
+    return upload_points(c);
+}
+
+int virs_set_receivers(virs_ctx *c, const int64_t *idx, int32_t nrcv)
+{
+    if (!c || nrcv < 0 || nrcv > kMaxPoints || (nrcv && !idx)) return VIRS_ERR_ARG;
+    for (int k = 0; k < nrcv; ++k)
+        if (idx[k] < 0 || idx[k] >= c->N) return fail(c, VIRS_ERR_ARG, "receiver %d index out of range", k);
+    HIPC(c, hipSetDevice(c->d.device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    c->nrcv = nrcv;
+    c->h_rcv_idx.assign(idx, idx + nrcv);
+    dfree(c->ir_buf);
+    c->ir_cap = 0;
+    drop_graphs(c);
+    return upload_points(c);
+}
+
+int virs_set_early_stop(virs_ctx *c, int32_t enable, float target)
+{
+    if (!c) return VIRS_ERR_ARG;
+    c->es_enable = enable;
+    c->es_target = target > 0.f ? target : 0.0009f;
+    return VIRS_OK;
+}
+
+int virs_start_simulation(virs_ctx *c, int64_t nsteps)
+{
+    if (!c || nsteps < 0) return VIRS_ERR_ARG;
+    HIPC(c, hipSetDevice(c->d.device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    dfree(c->ir_buf);
+    c->ir_cap = nsteps;
+    const size_t bytes = sizeof(double) * (size_t)std::max(1, c->nrcv) * (size_t)std::max<long long>(1, nsteps);
+    int rc = dalloc(c, &c->ir_buf, bytes);
+    if (rc) return rc;
+    HIPC(c, hipMemsetAsync(c->ir_buf, 0, bytes, c->stream));
+    rc = reset_run(c);
+    if (rc) return rc;
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return VIRS_OK;
+}
+
+static int build_graph(virs_ctx *c, int steps)
+{
+    const int phase = c->ip;
+    const int sp = c->ip, sq = c->iq, sr = c->ir;
+    hipGraph_t g = nullptr;
+    HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = VIRS_OK;
+    for (int s = 0; s < steps && !rc; ++s) rc = launch_step(c);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->ip = sp;
+    c->iq = sq;
+    c->ir = sr;
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(c, VIRS_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&c->graph[phase], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        c->graph[phase] = nullptr;
+        return fail(c, VIRS_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+    }
+    c->graph_steps = steps;
+    return VIRS_OK;
+}
+
+int virs_step(virs_ctx *c, int64_t nsteps, int32_t *keep_going)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (nsteps < 0) return VIRS_ERR_ARG;
+    if (keep_going) *keep_going = c->stopped ? 0 : 1;
+    if (c->stopped) return VIRS_OK;
+    const long long first = c->steps;
+    long long left = nsteps;
+    const int gsteps = 30; // a multiple of 3 keeps the buffer rotation phase
+    if (c->d.use_graph) {
+        while (left >= gsteps) {
+            if (!c->graph[c->ip] || c->graph_steps != gsteps) {
+                rc = build_graph(c, gsteps);
+                if (rc) return rc;
+            }
+            HIPC(c, hipGraphLaunch(c->graph[c->ip], c->stream));
+            left -= gsteps;
+            c->steps += gsteps;
+        }
+    }
+    for (; left > 0; --left) {
+        rc = launch_step(c);
+        if (rc) return rc;
+        ++c->steps;
+    }
+    c->counted = c->steps;
+    if (c->es_enable && c->nrcv > 0) {
+        // RMS rule of simulator_gpu.cu:303-330 on receiver 0, in float
+        const long long last = std::min<long long>(c->steps, c->ir_cap);
+        if (last > first) {
+            std::vector<double> tmp((size_t)(last - first));
+            HIPC(c, hipMemcpyAsync(tmp.data(), c->ir_buf + first, sizeof(double) * (last - first),
+                                   hipMemcpyDeviceToHost, c->stream));
+            HIPC(c, hipStreamSynchronize(c->stream));
+            for (double v : tmp) c->ir0_host.push_back((float)v);
+        }
+        const unsigned RMS_SIZE = 100, MIN_STEP = 3000;
+        for (long long s = first; s < last; ++s) {
+            const unsigned step = (unsigned)s;
+            if (step > MIN_STEP) {
+                float sum = 0.0f;
+                for (unsigned i = step - RMS_SIZE; i < step; ++i) sum += c->ir0_host[i] * c->ir0_host[i];
+                float rms = std::sqrt(sum / (float)RMS_SIZE);
+                if (step == MIN_STEP + 1) {
+                    c->rms_state = rms;
+                } else {
+                    const float fg = 0.00008f;
+                    const float vout = (fg * rms + c->rms_state) / (fg + 1.f);
+                    c->rms_state = 2.f * vout - c->rms_state;
+                    rms = vout;
+                }
+                c->percent = c->es_target / rms;
+                if (rms < c->es_target) {
+                    c->stopped = true;
+                    c->counted = s + 1;
+                    if (keep_going) *keep_going = 0;
+                    break;
+                }
+            }
+        }
+    }
+    return VIRS_OK;
+}
+
+int64_t virs_steps_done(const virs_ctx *c) { return c ? c->counted : -1; }
+
+float virs_percent_to_target_rms(const virs_ctx *c) { return c ? c->percent : 0.f; }
+
+int virs_synchronize(virs_ctx *c)
+{
+    if (!c) return VIRS_ERR_ARG;
+    HIPC(c, hipSetDevice(c->d.device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    int err = 0;
+    HIPC(c, hipMemcpy(&err, &c->ctl->err, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) return fail(c, VIRS_ERR_STATE, "non-finite field maximum: the run diverged");
+    return VIRS_OK;
+}
+
+int virs_read_receivers_f64(virs_ctx *c, double *out)
+{
+    if (!c || (!out && c->nrcv)) return VIRS_ERR_ARG;
+    HIPC(c, hipSetDevice(c->d.device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    const long long k = std::min(c->counted, c->ir_cap);
+    for (int r = 0; r < c->nrcv; ++r)
+        if (k) HIPC(c, hipMemcpy(out + (long long)r * k, c->ir_buf + (long long)r * c->ir_cap,
+                                 sizeof(double) * k, hipMemcpyDeviceToHost));
+    return VIRS_OK;
+}
+
+int virs_read_receivers(virs_ctx *c, float *out)
+{
+    if (!c) return VIRS_ERR_ARG;
+    const long long k = std::min(c->counted, c->ir_cap);
+    std::vector<double> tmp((size_t)std::max<long long>(1, k * c->nrcv));
+    int rc = virs_read_receivers_f64(c, tmp.data());
+    if (rc) return rc;
+    for (long long i = 0; i < k * c->nrcv; ++i) out[i] = (float)tmp[i];
+    return VIRS_OK;
+}
+
+int virs_field_exponent(virs_ctx *c, int32_t which, int32_t *e)
+{
+    if (!c || !e || which < 0 || which > 1) return VIRS_ERR_ARG;
+    *e = 0;
+    if (c->d.precision != VIRS_FP16 && c->d.precision != VIRS_BF16) return VIRS_OK;
+    HIPC(c, hipSetDevice(c->d.device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    StepCtl h;
+    HIPC(c, hipMemcpy(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost));
+    *e = h.exp[(h.n - 1 - which) & 3];
+    return VIRS_OK;
+}
+
+int virs_read_field_f64(virs_ctx *c, int32_t which, double *out)
+{
+    if (!c || !out || which < 0 || which > 1) return VIRS_ERR_ARG;
+    int e = 0;
+    int rc = virs_field_exponent(c, which, &e);
+    if (rc) return rc;
+    double *tmp = nullptr;
+    rc = dalloc(c, &tmp, sizeof(double) * c->N);
+    if (rc) return rc;
+    const void *src = c->f[which == 0 ? c->ip : c->iq];
+    const int g = 4096;
+    switch (c->d.precision) {
+    case VIRS_FP16: hipLaunchKernelGGL(k_to_f64<f16>, dim3(g), dim3(256), 0, c->stream, (const f16 *)src, tmp, c->N, e); break;
+    case VIRS_BF16: hipLaunchKernelGGL(k_to_f64<bf16>, dim3(g), dim3(256), 0, c->stream, (const bf16 *)src, tmp, c->N, e); break;
+    case VIRS_FP64: hipLaunchKernelGGL(k_to_f64<double>, dim3(g), dim3(256), 0, c->stream, (const double *)src, tmp, c->N, e); break;
+    default: hipLaunchKernelGGL(k_to_f64<float>, dim3(g), dim3(256), 0, c->stream, (const float *)src, tmp, c->N, e); break;
+    }
+    hipError_t err = hipGetLastError();
+    if (err == hipSuccess) err = hipMemcpyAsync(out, tmp, sizeof(double) * c->N, hipMemcpyDeviceToHost, c->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (err != hipSuccess) return fail(c, VIRS_ERR_HIP, "read field: %s", hipGetErrorString(err));
+    return VIRS_OK;
+}
+
+int virs_read_field(virs_ctx *c, int32_t which, float *out)
+{
+    if (!c || !out) return VIRS_ERR_ARG;
+    std::vector<double> tmp((size_t)c->N);
+    int rc = virs_read_field_f64(c, which, tmp.data());
+    if (rc) return rc;
+    for (long long i = 0; i < c->N; ++i) out[i] = (float)tmp[i];
+    return VIRS_OK;
+}
+
+int virs_write_field_f64(virs_ctx *c, int32_t which, const double *in)
+{
+    if (!c || !in || which < 0 || which > 1) return VIRS_ERR_ARG;
+    HIPC(c, hipSetDevice(c->d.device));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    int e = 0;
+    if (c->d.precision == VIRS_FP16 || c->d.precision == VIRS_BF16) {
+        // pick the exponent for the new field and record it where the step
+        // reads it: exp[(n-1)&3] for p^n, exp[(n-2)&3] for p^{n-1}
+        double mx = 0.0;
+        for (long long i = 0; i < c->N; ++i) mx = std::max(mx, std::fabs(in[i]));
+        if (mx > 0.0) {
+            int k2;
+            std::frexp(mx * 16.0, &k2);
+            e = std::min(std::max(14 - k2, -100), 100);
+        }
+        StepCtl h;
+        HIPC(c, hipMemcpy(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost));
+        const int slot = (int)((h.n - 1 - which) & 3);
+        h.exp[slot] = e;
+        const float fm = (float)mx;
+        unsigned bits;
+        std::memcpy(&bits, &fm, 4);
+        for (int s = 0; s < kSlots; ++s) h.maxabs[slot][s] = bits;
+        HIPC(c, hipMemcpy(c->ctl, &h, sizeof h, hipMemcpyHostToDevice));
+    }
+    double *tmp = nullptr;
+    int rc = dalloc(c, &tmp, sizeof(double) * c->N);
+    if (rc) return rc;
+    void *dst = c->f[which == 0 ? c->ip : c->iq];
+    hipError_t err = hipMemcpy(tmp, in, sizeof(double) * c->N, hipMemcpyHostToDevice);
+    const int g = 4096;
+    if (err == hipSuccess) {
+        switch (c->d.precision) {
+        case VIRS_FP16: hipLaunchKernelGGL(k_from_f64<f16>, dim3(g), dim3(256), 0, c->stream, tmp, (f16 *)dst, c->N, e); break;
+        case VIRS_BF16: hipLaunchKernelGGL(k_from_f64<bf16>, dim3(g), dim3(256), 0, c->stream, tmp, (bf16 *)dst, c->N, e); break;
+        case VIRS_FP64: hipLaunchKernelGGL(k_from_f64<double>, dim3(g), dim3(256), 0, c->stream, tmp, (double *)dst, c->N, e); break;
+        default: hipLaunchKernelGGL(k_from_f64<float>, dim3(g), dim3(256), 0, c->stream, tmp, (float *)dst, c->N, e); break;
+        }
+        err = hipGetLastError();
+    }
+    if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
+    (void)hipFree(tmp);
+    if (err != hipSuccess) return fail(c, VIRS_ERR_HIP, "write field: %s", hipGetErrorString(err));
+    return VIRS_OK;
+}
+
+int virs_memory_bytes(const virs_ctx *c, int64_t *bytes)
+{
+    if (!c || !bytes) return VIRS_ERR_ARG;
+    long long b = 3 * (long long)c->esz * c->N + c->N; // fields + cell codes
+    if (c->normals) b += c->N;
+    b += c->nb * (4 + 32 + (c->bmat ? 1 : 0));
+    b += (long long)sizeof(double) * std::max(1, c->nrcv) * std::max<long long>(1, c->ir_cap);
+    b += (long long)c->cp_field.size() * (long long)c->esz * c->N;
+    b += (long long)c->cp_state.size() * 32 * c->nb;
+    b += 2 * (long long)sizeof(double) * c->trace_slots * c->nb;
+    for (auto p : c->af)
+        if (p) b += (long long)c->esz * c->N;
+    *bytes = b;
+    return VIRS_OK;
+}
+
+int virs_time_steps(virs_ctx *c, int64_t nsteps, double *total_ms, double *dense_ms, double *boundary_ms)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (nsteps <= 0) return VIRS_ERR_ARG;
+    std::vector<hipEvent_t> ev((size_t)nsteps * 3);
+    for (auto &e : ev) HIPC(c, hipEventCreate(&e));
+    hipEvent_t t0, t1;
+    HIPC(c, hipEventCreate(&t0));
+    HIPC(c, hipEventCreate(&t1));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    HIPC(c, hipEventRecord(t0, c->stream));
+    for (long long s = 0; s < nsteps && !rc; ++s) {
+        rc = launch_step(c, nullptr, nullptr, 0, true, &ev[(size_t)s * 3]);
+        ++c->steps;
+    }
+    c->counted = c->steps;
+    HIPC(c, hipEventRecord(t1, c->stream));
+    HIPC(c, hipEventSynchronize(t1));
+    double dsum = 0, bsum = 0;
+    for (long long s = 0; s < nsteps; ++s) {
+        float a = 0, b = 0;
+        HIPC(c, hipEventElapsedTime(&a, ev[s * 3], ev[s * 3 + 1]));
+        HIPC(c, hipEventElapsedTime(&b, ev[s * 3 + 1], ev[s * 3 + 2]));
+        dsum += a;
+        bsum += b;
+    }
+    float tot = 0;
+    HIPC(c, hipEventElapsedTime(&tot, t0, t1));
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (rc) return rc;
+    if (total_ms) *total_ms = tot;
+    if (dense_ms) *dense_ms = dsum / (double)nsteps;
+    if (boundary_ms) *boundary_ms = bsum / (double)nsteps;
+    return VIRS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint with checkpointed recomputation.
+//
+// Forward: snapshots of (p^n, p^{n-1}, filter state) at the start of every
+// segment of `seg` steps; the last segment records its boundary trace
+// (d^n, y^n per entry) directly.  Reverse, segment by segment from the end:
+// restore the snapshot, recompute the segment forward recording the trace,
+// then run the adjoint steps of the segment backwards.  Cost: one forward,
+// one recomputation of all but the last segment, one adjoint sweep.
+// ---------------------------------------------------------------------------
+
+static int ensure_trace(virs_ctx *c, long long slots)
+{
+    if (c->trace_slots >= slots && c->trace_d) return VIRS_OK;
+    dfree(c->trace_d);
+    dfree(c->trace_y);
+    c->trace_slots = 0;
+    int rc = dalloc(c, &c->trace_d, sizeof(double) * slots * std::max<long long>(1, c->nb));
+    if (!rc) rc = dalloc(c, &c->trace_y, sizeof(double) * slots * std::max<long long>(1, c->nb));
+    if (rc) return rc;
+    c->trace_slots = slots;
+    return VIRS_OK;
+}
+
+static int seed_trace(virs_ctx *c)
+{
+    if (!c->nb) return VIRS_OK;
+    hipLaunchKernelGGL(k_trace_seed, dim3(grid_for(c->nb)), dim3(256), 0, c->stream, c->st[0], c->st[1],
+                       c->st[2], c->st[3], c->trace_d, c->trace_y, c->nb);
+    HIPC(c, hipGetLastError());
+    return VIRS_OK;
+}
+
+int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (c->d.precision != VIRS_FP32 && c->d.precision != VIRS_FP64)
+        return fail(c, VIRS_ERR_UNSUPPORTED, "adjoint needs FP32 or FP64 fields");
+    if (c->nmat > 8) return fail(c, VIRS_ERR_UNSUPPORTED, "adjoint supports up to 8 materials");
+    if (nsteps <= 0 || ncp <= 0) return VIRS_ERR_ARG;
+    if (c->ir_cap < nsteps) return fail(c, VIRS_ERR_STATE, "receiver buffer shorter than nsteps");
+    ncp = (int)std::min<long long>(ncp, nsteps);
+    c->seg = (nsteps + ncp - 1) / ncp;
+    ncp = (int)((nsteps + c->seg - 1) / c->seg);
+    // (re)allocate snapshots
+    if ((int)c->cp_state.size() != ncp) {
+        for (auto &p : c->cp_field) dfree(p);
+        for (auto &p : c->cp_state) dfree(p);
+        c->cp_field.assign((size_t)ncp * 2, nullptr);
+        c->cp_state.assign((size_t)ncp, nullptr);
+        for (int k = 0; k < ncp && !rc; ++k) {
+            rc = dalloc(c, &c->cp_field[2 * k], c->esz * (size_t)c->N);
+            if (!rc) rc = dalloc(c, &c->cp_field[2 * k + 1], c->esz * (size_t)c->N);
+            if (!rc) rc = dalloc(c, &c->cp_state[k], sizeof(double) * 4 * std::max<long long>(1, c->nb));
+        }
+        if (rc) return rc;
+    }
+    c->ncp = ncp;
+    rc = ensure_trace(c, c->seg + 2);
+    if (rc) return rc;
+    rc = reset_run(c);
+    if (rc) return rc;
+    const long long last_base = (long long)(ncp - 1) * c->seg;
+    for (long long s = 0; s < nsteps; ++s) {
+        if (s % c->seg == 0) {
+            const int k = (int)(s / c->seg);
+            HIPC(c, hipMemcpyAsync(c->cp_field[2 * k], c->f[c->ip], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
+            HIPC(c, hipMemcpyAsync(c->cp_field[2 * k + 1], c->f[c->iq], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
+            for (int j = 0; j < 4 && c->nb; ++j)
+                HIPC(c, hipMemcpyAsync(c->cp_state[k] + j * c->nb, c->st[j], sizeof(double) * c->nb,
+                                       hipMemcpyDeviceToDevice, c->stream));
+            if (s == last_base) {
+                rc = seed_trace(c);
+                if (rc) return rc;
+            }
+        }
+        if (s >= last_base) rc = launch_step(c, c->trace_d, c->trace_y, last_base, true);
+        else rc = launch_step(c);
+        if (rc) return rc;
+        ++c->steps;
+    }
+    c->counted = c->steps;
+    c->rec_steps = nsteps;
+    c->last_seg_base = last_base;
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return VIRS_OK;
+}
+
+int virs_adjoint(virs_ctx *c, const float *dl_dir, double *grad)
+{
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!grad || (c->nrcv && !dl_dir)) return VIRS_ERR_ARG;
+    if (c->rec_steps <= 0 || c->ncp <= 0) return fail(c, VIRS_ERR_STATE, "virs_forward_record first");
+    const long long T = c->rec_steps;
+    // adjoint buffers
+    for (int k = 0; k < 3 && !rc; ++k)
+        if (!c->af[k]) rc = dalloc(c, &c->af[k], c->esz * (size_t)c->N);
+    for (int k = 0; k < 4 && !rc; ++k)
+        if (!c->ast[k]) rc = dalloc(c, &c->ast[k], sizeof(double) * std::max<long long>(1, c->nb));
+    if (rc) return rc;
+    dfree(c->gsig);
+    rc = dalloc(c, &c->gsig, sizeof(double) * std::max(1, c->nrcv) * T);
+    if (rc) return rc;
+    {
+        std::vector<double> g((size_t)std::max(1, c->nrcv) * T, 0.0);
+        for (long long i = 0; i < (long long)c->nrcv * T; ++i) g[i] = dl_dir[i];
+        HIPC(c, hipMemcpy(c->gsig, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice));
+    }
+    const long long gb = tail_grid(c);
+    if (c->grad_blocks != gb || !c->grad_part) {
+        dfree(c->grad_part);
+        rc = dalloc(c, &c->grad_part, sizeof(double) * 5 * 8 * gb);
+        if (rc) return rc;
+        c->grad_blocks = gb;
+    }
+    HIPC(c, hipMemsetAsync(c->grad_part, 0, sizeof(double) * 5 * 8 * gb, c->stream));
+    rc = zero_fields(c, c->af);
+    if (rc) return rc;
+    for (int k = 0; k < 4; ++k)
+        if (c->nb) HIPC(c, hipMemsetAsync(c->ast[k], 0, sizeof(double) * c->nb, c->stream));
+    hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl_adj, T - 1, 1);
+    HIPC(c, hipGetLastError());
+    int ap = 0, aq = 1, ar = 2;
+    for (int k = c->ncp - 1; k >= 0; --k) {
+        const long long s0 = (long long)k * c->seg, s1 = std::min(T, s0 + c->seg);
+        if (s0 != c->last_seg_base) {
+            // restore the snapshot into the forward buffers and recompute
+            c->ip = 0;
+            c->iq = 1;
+            c->ir = 2;
+            HIPC(c, hipMemcpyAsync(c->f[c->ip], c->cp_field[2 * k], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
+            HIPC(c, hipMemcpyAsync(c->f[c->iq], c->cp_field[2 * k + 1], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
+            for (int j = 0; j < 4 && c->nb; ++j)
+                HIPC(c, hipMemcpyAsync(c->st[j], c->cp_state[k] + j * c->nb, sizeof(double) * c->nb,
+                                       hipMemcpyDeviceToDevice, c->stream));
+            hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl, s0, 0);
+            HIPC(c, hipGetLastError());
+            rc = seed_trace(c);
+            if (rc) return rc;
+            for (long long s = s0; s < s1; ++s) {
+                rc = launch_step(c, c->trace_d, c->trace_y, s0, false);
+                if (rc) return rc;
+            }
+            c->last_seg_base = s0;
+        }
+        for (long long s = s1 - 1; s >= s0; --s) {
+            int grid = 0;
+            DenseParams p = dense_params(c, c->af[ap], c->af[aq], c->af[ar], c->ctl_adj, grid);
+            p.nsoft = 0;
+            HIPC(c, launch_dense(c, p, grid, true));
+            TailParams t = tail_params(c, c->af[ap], c->af[aq], c->af[ar], c->ctl_adj);
+            t.s0 = c->ast[0];
+            t.s1 = c->ast[1];
+            t.s2 = c->ast[2];
+            t.s3 = c->ast[3];
+            t.ir = nullptr;
+            t.gsig = c->gsig;
+            t.ir_cap = T;
+            t.nsoft = 0;
+            t.trace_d = c->trace_d;
+            t.trace_y = c->trace_y;
+            t.trace_base = s0;
+            t.trace_slots = c->trace_slots;
+            t.grad_part = c->grad_part;
+            HIPC(c, launch_tail_adj(c, t));
+            const int tp = ap;
+            ap = ar;
+            ar = aq;
+            aq = tp;
+        }
+    }
+    std::vector<double> part((size_t)5 * 8 * gb);
+    HIPC(c, hipMemcpyAsync(part.data(), c->grad_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < 5 * c->nmat; ++i) grad[i] = 0.0;
+    for (long long b = 0; b < gb; ++b)
+        for (int i = 0; i < 5 * c->nmat; ++i) grad[i] += part[(size_t)b * 5 * c->nmat + i];
+    // leave the forward state consistent with a finished run
+    c->last_seg_base = -1;
+    c->rec_steps = 0;
+    return VIRS_OK;
+}
+
+} // extern "C"
