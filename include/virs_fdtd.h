@@ -57,10 +57,14 @@ typedef struct {
     float sponge_sigma;     /* damping per step at the outermost cell */
     int32_t device;         /* HIP device ordinal */
     int32_t use_graph;      /* replay batches of steps as HIP graphs */
-    int32_t zchunks;        /* z-march chunks of the stencil launch; 0 = automatic */
+    int32_t dense_blocks;   /* stencil grid size; 0 = fill the GPU once */
+    int32_t fuse_boundary;  /* 1: run the boundary biquad inside the stencil
+                               pass when the list is the grid's interior
+                               flag-2 cells; 0: always a separate list launch */
 } virs_desc;
 
-/* Fills the reference defaults: c = 343, dt = 0, FP32, no sponge, device 0. */
+/* Fills the reference defaults: c = 343, dt = 0, FP32, no sponge, device 0,
+ * boundary fused into the stencil pass. */
 int virs_desc_default(virs_desc *d);
 
 /* fdtd_gpu_setup (simulator_gpu.cpp:7-235): allocate the device grid, zeroed. */
@@ -91,7 +95,8 @@ int virs_set_materials(virs_ctx *ctx, const double *coeffs, int32_t nmat);
  * pulse: 10 for steps 0-9, then 0 (simulator_gpu.cu:214-218). */
 int virs_set_hard_source(virs_ctx *ctx, const double *sig, int64_t len);
 /* Soft (additive) point sources, no reference counterpart: after the
- * stencil, p[idx[k]] += sig[k][n].  Cells must be distinct. */
+ * stencil and the boundary correction, p[idx[k]] += sig[k][n].  Cells must
+ * be distinct. */
 int virs_set_soft_sources(virs_ctx *ctx, const int64_t *idx, const double *sig, int32_t nsrc,
                           int64_t len);
 /* Receiver voxels.  The reference has one, fixed at (334,225,270)

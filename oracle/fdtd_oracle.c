@@ -14,7 +14,8 @@
  *     constants         src/simulator/simulator_gpu.cu:195-198,147
  *   Extensions of this build (no reference counterpart), each reducing to the
  *   reference when switched off: per-voxel material rows, FP64 filter state,
- *   soft point sources, several receivers, graded sponge damping.
+ *   soft point sources (added after the boundary correction), several
+ *   receivers, graded sponge damping.
  *   oracle_rms_continue   src/simulator/simulator_gpu.cu:303-330
  *   oracle_refcpu_steps   src/simulator/simulator_cpu.cpp:55-81 with
  *                         Grid::laplacian7 src/simulator/vectorspace.h:107-134
@@ -155,11 +156,6 @@ int oracle_run_##SUFFIX(const oracle_desc *d, int64_t n0, int64_t nsteps, double
                 }                                                                         \
             }                                                                             \
         }                                                                                 \
-        for (int k = 0; k < d->nsoft; ++k) {                                              \
-            const int64_t si = d->soft_idx[k];                                            \
-            const double v = n < d->soft_len ? d->soft_sig[(int64_t)k * d->soft_len + n] : 0.0; \
-            R[si] = R[si] + (T)v;                                                         \
-        }                                                                                 \
         for (int64_t e = 0; e < d->nb; ++e) {                                             \
             const int64_t bi = d->bidx[e];                                                \
             const int m = d->bmat ? d->bmat[e] : 0;                                       \
@@ -185,6 +181,11 @@ int oracle_run_##SUFFIX(const oracle_desc *d, int64_t n0, int64_t nsteps, double
             if (trace_y) trace_y[s * d->nb + e] = yv;                                     \
             R[bi] = (T)((double)R[bi] -                                                   \
                         (double)d->kconst * (yv * 0.1 + (double)dif * 0.9) * (double)0.4f); \
+        }                                                                                 \
+        for (int k = 0; k < d->nsoft; ++k) {                                              \
+            const int64_t si = d->soft_idx[k];                                            \
+            const double v = n < d->soft_len ? d->soft_sig[(int64_t)k * d->soft_len + n] : 0.0; \
+            R[si] = R[si] + (T)v;                                                         \
         }                                                                                 \
         T *t = Q; Q = P; P = R; R = t;                                                    \
         for (int r = 0; r < d->nrcv; ++r) ir[(int64_t)r * nsteps + s] = (double)P[d->rcv_idx[r]]; \

@@ -1,0 +1,93 @@
+"""Adjoint gradients on the GPU against the oracle's reverse-mode transpose
+(itself pinned by central finite differences in test_oracle.py) and against
+central differences of the GPU forward."""
+import numpy as np
+import pytest
+
+from helpers import gpu_context, oracle_problem, rel_l2, tiny_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _loss_grad(f, n, target):
+    f.forward_record(n, 1)
+    ir = f.receivers(True)
+    return float(np.sum((ir - target) ** 2)), 2 * (ir - target)
+
+
+@pytest.mark.parametrize("sponge", [0, 3])
+def test_fp64_adjoint_matches_oracle(gpu, sponge):
+    p = tiny_problem(21, dims=(20, 16, 12), steps=90, sponge=sponge)
+    n = p["steps"]
+    target = np.random.default_rng(1).standard_normal((len(p["rcv_idx"]), n)) * 1e-3
+    f = gpu_context(p, "fp64")
+    f.forward_record(n, 5)
+    ir = f.receivers(True)
+    ref_ir = oracle_problem(p).run(n, "f64")["ir"]
+    np.testing.assert_array_equal(ir, ref_ir)
+    dl = 2 * (ir - target)
+    g = f.adjoint(dl.astype(np.float32))
+    g_or = oracle_problem(p).adjoint(n, dl.astype(np.float32).astype(np.float64))
+    print(f"fp64 adjoint vs oracle rel {rel_l2(g, g_or):.2e}", flush=True)
+    assert rel_l2(g, g_or) <= 1e-10
+    f.close()
+
+
+def test_fp32_adjoint_within_1e4_of_fp64(gpu):
+    p = tiny_problem(22, dims=(22, 16, 14), steps=100, sponge=2)
+    n = p["steps"]
+    target = np.random.default_rng(2).standard_normal((len(p["rcv_idx"]), n)) * 1e-3
+    f64 = gpu_context(p, "fp64")
+    f64.forward_record(n, 4)
+    dl = (2 * (f64.receivers(True) - target)).astype(np.float32)
+    g64 = f64.adjoint(dl)
+    f32 = gpu_context(p, "fp32")
+    f32.forward_record(n, 4)
+    g32 = f32.adjoint(dl)
+    e = rel_l2(g32, g64)
+    print(f"fp32 adjoint vs fp64 rel {e:.2e}", flush=True)
+    assert e <= 1e-4
+    f64.close()
+    f32.close()
+
+
+@pytest.mark.parametrize("ncp", [1, 2, 7, 100])
+def test_checkpoint_count_does_not_change_the_gradient(gpu, ncp):
+    p = tiny_problem(23, dims=(18, 14, 12), steps=100)
+    n = p["steps"]
+    dl = np.random.default_rng(3).standard_normal((len(p["rcv_idx"]), n)).astype(np.float32)
+    f = gpu_context(p, "fp64")
+    f.forward_record(n, 1)
+    ref = f.adjoint(dl)
+    f.forward_record(n, ncp)
+    g = f.adjoint(dl)
+    np.testing.assert_array_equal(g, ref)
+    f.close()
+
+
+def test_fp64_adjoint_matches_central_differences(gpu):
+    p = tiny_problem(24, dims=(16, 14, 12), steps=70, sponge=2)
+    n = p["steps"]
+    target = np.random.default_rng(4).standard_normal((len(p["rcv_idx"]), n)) * 1e-3
+    f = gpu_context(p, "fp64")
+    L0, dl = _loss_grad(f, n, target)
+    f.forward_record(n, 3)
+    g = f.adjoint(dl.astype(np.float32))
+    c0 = p["coeffs"].copy()
+    h = 1e-3
+    fd = np.zeros_like(c0)
+    for m in range(c0.shape[0]):
+        for k in range(5):
+            vals = []
+            for sgn in (1, -1):
+                c = c0.copy()
+                c[m, k] += sgn * h
+                f.set_materials(c)
+                vals.append(_loss_grad(f, n, target)[0])
+            fd[m, k] = (vals[0] - vals[1]) / (2 * h)
+    f.set_materials(c0)
+    e = rel_l2(g, fd)
+    print(f"fp64 adjoint vs central differences rel {e:.2e}", flush=True)
+    # dl was rounded to float32 for the adjoint call
+    assert e <= 1e-5
+    f.close()

@@ -18,12 +18,13 @@ def _say(msg):
     print(msg, flush=True)
 
 
+@pytest.mark.parametrize("fuse", [True, False])
 @pytest.mark.parametrize("seed", [0, 1, 2])
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
-def test_tiny_bit_exact(gpu, seed, precision):
+def test_tiny_bit_exact(gpu, seed, precision, fuse):
     p = tiny_problem(seed)
-    _say(f"tiny seed={seed} {precision} dims={p['dims']} nb={len(p['bidx'])}")
-    f = gpu_context(p, precision)
+    _say(f"tiny seed={seed} {precision} fuse={fuse} dims={p['dims']} nb={len(p['bidx'])}")
+    f = gpu_context(p, precision, fuse=fuse)
     f.step(p["steps"])
     f.synchronize()
     ir = f.receivers(f64=True)
@@ -66,13 +67,14 @@ def test_graph_replay_matches_direct(gpu):
     b.close()
 
 
-def test_repeatable_and_zchunks_invariant(gpu):
-    """Jacobi-style update: two runs, and two different launch shapes, give
+def test_repeatable_and_launch_shape_invariant(gpu):
+    """Jacobi-style update: repeated runs and different stencil grid sizes
+    (one block marching everything, odd splits, one plane per block) give
     bitwise identical results (the race check)."""
     p = tiny_problem(4, dims=(32, 20, 40), steps=80)
     outs = []
-    for zc in (0, 1, 7, 40):
-        f = gpu_context(p, zchunks=zc)
+    for nblk in (0, 0, 1, 7, 40, 100000):
+        f = gpu_context(p, dense_blocks=nblk)
         f.step(p["steps"])
         outs.append((f.receivers(True), f.field(0)))
         f.close()
@@ -81,9 +83,10 @@ def test_repeatable_and_zchunks_invariant(gpu):
         np.testing.assert_array_equal(fld, outs[0][1])
 
 
-@pytest.mark.parametrize("nx", [23, 26, 30])
+@pytest.mark.parametrize("nx", [23, 26, 30, 64, 136])
 def test_odd_widths(gpu, nx):
-    """Row widths that force 1- and 2-wide vectors along x."""
+    """Row widths that force 1-, 2- and 4-wide vectors along x, and planes
+    that span several 256-thread chunks."""
     p = tiny_problem(5, dims=(nx, 15, 13), steps=40)
     for prec in ("fp32", "fp64"):
         f = gpu_context(p, prec)

@@ -2,8 +2,9 @@
 //
 // Semantics follow the reference GPU step (src/simulator/simulator_gpu.cu):
 //   dense update      :38-103   -> k_dense
-//   boundary biquad   :105-150  -> k_tail (with point sources, receivers and
-//                                  the device step counter)
+//   boundary biquad   :105-150  -> fused into k_dense (per-plane slices of the
+//                                  compact list), or k_tail over the list
+//   source, receiver  :59-63,214-218,292-300 -> k_dense (hard) / k_tail (points)
 //   adjoint           (new)     -> k_dense<ADJ> + k_tail_adj
 // Arithmetic is written in the reference's operand order and the file is
 // compiled without FP contraction, so an FP32/FP64 run is bit-identical to a
@@ -18,18 +19,22 @@
 namespace virs {
 
 constexpr int kBlock = 256;   // 4 waves of 64
+constexpr int kWaves = kBlock / 64;
 constexpr int kSlots = 64;    // spread of the per-step |p| maximum atomics
 constexpr int kMaxSponge = 64;
 constexpr int kMaxPoints = 64;
 
-// Device-resident step state.  n is the step being computed; exp/maxabs are
-// rings indexed by step & 3 holding each stored field's power-of-two scale
-// exponent and its max |value| (physical units, float bits; non-negative
-// floats order like their bit patterns).
+// Device-resident step state.  nring[k] is the step index that the launches
+// of rotation phase k compute; each step's last launch writes the next
+// phase's entry, which no block of the current launch reads, so the counter
+// advances without atomics and a captured graph of whole rotation cycles
+// replays correctly.  exp/maxabs are rings indexed by step & 3 holding each
+// stored field's power-of-two scale exponent and max |value| (physical
+// units, float bits: non-negative floats order like their bit patterns).
 struct StepCtl {
-    long long n;
-    unsigned int done;
+    long long nring[3];
     int err;
+    int pad;
     int exp[4];
     unsigned int maxabs[4][kSlots];
 };
@@ -87,6 +92,54 @@ struct alignas(sizeof(T) * V) Vec {
     T v[V];
 };
 
+// Raw vector memory access of B bytes, optionally non-temporal (streamed
+// data: read or written once per step, kept out of the way of the reused
+// p^n planes in L2).
+template <int B>
+struct Raw;
+template <>
+struct Raw<16> {
+    typedef unsigned int type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Raw<8> {
+    typedef unsigned int type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct Raw<4> {
+    typedef unsigned int type;
+};
+template <>
+struct Raw<2> {
+    typedef unsigned short type;
+};
+template <>
+struct Raw<1> {
+    typedef unsigned char type;
+};
+
+template <typename VT, bool NT>
+__device__ inline VT vload(const VT *p)
+{
+    using R = typename Raw<sizeof(VT)>::type;
+    R r;
+    if constexpr (NT) r = __builtin_nontemporal_load(reinterpret_cast<const R *>(p));
+    else r = *reinterpret_cast<const R *>(p);
+    VT v;
+    __builtin_memcpy(&v, &r, sizeof(VT));
+    return v;
+}
+
+template <typename VT, bool NT>
+__device__ inline void vstore(VT *p, const VT &v)
+{
+    using R = typename Raw<sizeof(VT)>::type;
+    R r;
+    __builtin_memcpy(&r, &v, sizeof(VT));
+    if constexpr (NT) __builtin_nontemporal_store(r, reinterpret_cast<R *>(p));
+    else *reinterpret_cast<R *>(p) = r;
+}
+
 __device__ inline float exp2i(int e) { return __int_as_float((e + 127) << 23); }
 
 template <typename C>
@@ -107,10 +160,10 @@ __device__ inline C wave_sum(C v)
 
 // Cell code, one byte per cell, built on the device from the reference flags:
 //   bits 0-5  neighbour is a wall (flag 1): +x 1, -x 2, +y 4, -y 8, +z 16, -z 32
-//   bits 6-7  0 update (flags 0 and 2), 1 write zero (flag 1 and the domain
-//             border), 2 hard source (flag 3)
-// The stencil then reads one byte per cell instead of seven flags.
-enum : uint8_t { kTypeUpdate = 0, kTypeZero = 1, kTypeHard = 2 };
+//   bits 6-7  0 update (flag 0), 3 update + boundary filter (interior flag 2),
+//             1 write zero (flag 1 and the domain border), 2 hard source (flag 3)
+// The stencil reads this one byte per cell instead of seven flags.
+enum : uint8_t { kTypeUpdate = 0, kTypeZero = 1, kTypeHard = 2, kTypeBoundary = 3 };
 
 __global__ void k_build_code(const uint8_t *__restrict__ flags, uint8_t *__restrict__ code, int nx,
                              int ny, int nz)
@@ -132,7 +185,7 @@ __global__ void k_build_code(const uint8_t *__restrict__ flags, uint8_t *__restr
             } else if (f == 3) {
                 c = kTypeHard << 6;
             } else {
-                c = 0;
+                c = f == 2 ? (uint8_t)(kTypeBoundary << 6) : (uint8_t)0;
                 if (flags[i + 1] == 1) c |= 1;
                 if (flags[i - 1] == 1) c |= 2;
                 if (flags[i + nx] == 1) c |= 4;
@@ -155,6 +208,17 @@ __device__ inline int edge_dist(int x, int y, int z, int nx, int ny, int nz)
     return min(min(min(x, nx - 1 - x), min(y, ny - 1 - y)), min(z, nz - 1 - z));
 }
 
+struct BoundaryState {
+    long long nb;
+    const uint8_t *bmat;       // may be null: row 0
+    double *s0, *s1, *s2, *s3; // forward: x1 x2 y1 y2; reverse: ybar1 ybar2 dbar1 dbar2
+    const double *coeffs;      // [nmat][5] b0 b1 b2 a1 a2
+    int nmat;
+    float kconst;              // (c*dt)/h*0.95f
+    double *trace_d, *trace_y; // [slots][nb], slot = n - trace_base + 2; null = off
+    long long trace_base, trace_slots;
+};
+
 struct DenseParams {
     const void *P;   // p^n      (forward) / adjoint of p^{n+1} (reverse)
     const void *Q;   // p^{n-1}  (forward) / adjoint of p^{n+2} (reverse)
@@ -163,10 +227,10 @@ struct DenseParams {
     int nx, ny, nz;
     long long sxy;
     long long nvec_plane; // vectors of V cells per z-plane
-    int nchunks_plane;    // blocks per z-plane
-    int zc;               // z-planes per block
+    int nchunks_plane;    // 256-thread chunks per z-plane
     float c2dt2, ih2;
     StepCtl *ctl;
+    int slot;             // rotation phase: reads ctl->nring[slot]
     const double *hard_sig;
     long long hard_len;
     int hard_ref;         // 1: the reference pulse (10 for n < 10)
@@ -174,15 +238,48 @@ struct DenseParams {
     const double *soft_sig;
     long long soft_len;
     Sponge sp;
+    // fused boundary filter: entries of (chunk, z) are
+    // [bstart[z*nchunks_plane + chunk], bstart[... + 1]) of the compact list
+    const long long *bstart; // null: the boundary list runs in k_tail
+    BoundaryState bs;
 };
+
+// DF1 biquad step and boundary correction for list entry e (simulator_gpu.cu:126-148):
+//   y = b0 d + b1 x1 + b2 x2 - a1 y1 - a2 y2   (FP64 state)
+//   R = R - K (0.1 y + 0.9 d) 0.4f             (evaluated in double)
+template <typename C>
+__device__ inline C boundary_update(const BoundaryState &bs, long long e, long long n, C dif, C r)
+{
+    const int m = bs.bmat ? bs.bmat[e] : 0;
+    const double *cf = bs.coeffs + 5 * m;
+    const double in = (double)dif;
+    const double x1 = bs.s0[e], x2 = bs.s1[e], y1 = bs.s2[e], y2 = bs.s3[e];
+    const double yv = cf[0] * in + cf[1] * x1 + cf[2] * x2 - cf[3] * y1 - cf[4] * y2;
+    bs.s1[e] = x1;
+    bs.s0[e] = in;
+    bs.s3[e] = y1;
+    bs.s2[e] = yv;
+    if (bs.trace_d) {
+        const long long slot = n - bs.trace_base + 2;
+        if (slot >= 0 && slot < bs.trace_slots) {
+            bs.trace_d[slot * bs.nb + e] = in;
+            bs.trace_y[slot * bs.nb + e] = yv;
+        }
+    }
+    return (C)((double)r - (double)bs.kconst * (yv * 0.1 + in * 0.9) * (double)0.4f);
+}
 
 // Leapfrog 7-point update, simulator_gpu.cu:38-103 with a graded sponge:
 //   lap = (V(x+1)+V(x-1)+V(y+1)+V(y-1)+V(z+1)+V(z-1) - 6 pc) * ih2
 //   R   = (a pc - b Q) + g (c2dt2 lap),   V(nbr) = pc where nbr is a wall
-// Each thread owns V consecutive x cells of one (y) row of the flattened
-// z-plane and marches up a chunk of z: the z neighbours stay in registers,
-// x neighbours come from the vector and from adjacent lanes, y neighbours are
-// the rows +-Nx of the same plane (cache hits from the neighbouring waves).
+// A thread owns V consecutive x cells of a row of the flattened z-plane and
+// marches up in z: z neighbours stay in registers, x neighbours come from
+// the vector and the adjacent lanes, y neighbours are the rows +-Nx of the
+// same plane (hits in L2 from the neighbouring chunks).  The next plane's
+// loads are issued before the current plane is computed.
+// Work split: the (chunk, z) planes, flattened chunk-major, are divided into
+// gridDim contiguous ranges of equal length, so every block marches ~the
+// same number of planes and the grid fits the GPU in one round.
 // ADJ: the reverse sweep applies the same operator to the adjoint field;
 // hard-source cells hold prescribed values, so their adjoint is zero.
 template <typename T, int V, bool ADJ>
@@ -193,24 +290,14 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
     using VT = Vec<T, V>;
     using VB = Vec<uint8_t, V>;
     const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
 
-    // XCD-aware order: the hardware places block b on XCD b % 8, so give each
-    // XCD a contiguous run of (z-chunk, plane-chunk) tiles and the y/z halo
-    // rows of neighbouring tiles meet in the same L2.
+    // XCD-aware order: the hardware places block b on XCD b % 8; give each
+    // XCD a contiguous run of logical ranges so neighbouring chunks share an L2.
     const unsigned nbk = gridDim.x, bid = blockIdx.x;
     const unsigned logical = (nbk % 8u == 0u) ? (bid % 8u) * (nbk / 8u) + bid / 8u : bid;
-    const int pchunk = (int)(logical % (unsigned)p.nchunks_plane);
-    const int zchunk = (int)(logical / (unsigned)p.nchunks_plane);
 
-    const long long v = (long long)pchunk * kBlock + threadIdx.x;
-    const bool active = v < p.nvec_plane;
-    const long long cell0 = active ? v * V : 0;
-    const int y = (int)(cell0 / p.nx);
-    const int x0 = (int)(cell0 - (long long)y * p.nx);
-    const int z0 = zchunk * p.zc;
-    const int z1 = min(z0 + p.zc, p.nz);
-
-    const long long n = p.ctl->n;
+    const long long n = p.ctl->nring[p.slot];
     C hard = (C)0;
     if (!ADJ) {
         if (p.hard_ref) hard = n < 10 ? (C)10 : (C)0;
@@ -247,109 +334,190 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
     const T *__restrict__ Q = (const T *)p.Q;
     T *__restrict__ R = (T *)p.R;
     const long long sxy = p.sxy;
-    const int nx = p.nx;
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
     const C c2dt2 = (C)p.c2dt2, ih2 = (C)p.ih2;
+    const bool fused = !ADJ && p.bstart != nullptr;
 
-    C pm[V], pc[V], pp[V];
-    auto loadP = [&](long long idx, C *out) {
-        const VT t = *reinterpret_cast<const VT *>(P + idx);
+    __shared__ float sp_tab[3][kMaxSponge];
+    __shared__ int wsum[2][kWaves];
+    const bool sponge = p.sp.w > 0;
+    if (sponge) {
+        for (int k = threadIdx.x; k < p.sp.w; k += kBlock) {
+            sp_tab[0][k] = p.sp.a[k];
+            sp_tab[1][k] = p.sp.b[k];
+            sp_tab[2][k] = p.sp.g[k];
+        }
+        __syncthreads();
+    }
+    int bparity = 0;
+    float lmax = 0.f;
+
+    const long long W = (long long)p.nchunks_plane * nz;
+    const long long per = W / nbk, rem = W % nbk;
+    const long long wbeg = (long long)logical * per + min((long long)logical, rem);
+    const long long wend = wbeg + per + ((long long)logical < rem ? 1 : 0);
+
+    auto cvt = [&](const VT &t, C *out, C s) {
 #pragma unroll
         for (int j = 0; j < V; ++j) {
-            if constexpr (S::kScaled) out[j] = S::ld(t.v[j]) * sP;
+            if constexpr (S::kScaled) out[j] = S::ld(t.v[j]) * s;
             else out[j] = S::ld(t.v[j]);
         }
     };
-#pragma unroll
-    for (int j = 0; j < V; ++j) pm[j] = pc[j] = (C)0;
-    if (active && z0 >= 1) loadP((long long)(z0 - 1) * sxy + cell0, pm);
-    if (active && z0 < p.nz) loadP((long long)z0 * sxy + cell0, pc);
 
-    float lmax = 0.f;
-    for (int z = z0; z < z1; ++z) {
-        const long long base = (long long)z * sxy + cell0;
-        C qv[V], yn[V], ys[V];
-        uint8_t cd[V];
+    for (long long w = wbeg; w < wend;) {
+        const int pchunk = (int)(w / nz);
+        const int z0 = (int)(w - (long long)pchunk * nz);
+        const int z1 = (int)min((long long)nz, (long long)z0 + (wend - w));
+        w += z1 - z0;
+
+        const long long v = (long long)pchunk * kBlock + threadIdx.x;
+        const bool active = v < p.nvec_plane;
+        const long long cell0 = active ? v * V : 0;
+        const int y = (int)(cell0 / nx);
+        const int x0 = (int)(cell0 - (long long)y * nx);
+        const bool has_yn = active && y + 1 < ny, has_ys = active && y > 0;
+        const bool need_l = active && lane == 0 && x0 > 0;
+        const bool need_r = active && lane == 63 && x0 + V < nx;
+
+        // registers: pm/pc/pp = P at z-1, z, z+1; the t* values are the raw
+        // loads for the plane being computed next
+        C pm[V], pc[V];
+        VT tpp{}, tq{}, tyn{}, tys{};
+        VB tcd{};
+        T tl{}, tr{};
 #pragma unroll
-        for (int j = 0; j < V; ++j) pp[j] = qv[j] = yn[j] = ys[j] = (C)0, cd[j] = kTypeZero << 6;
+        for (int j = 0; j < V; ++j) pm[j] = pc[j] = (C)0;
         if (active) {
-            if (z + 1 < p.nz) loadP(base + sxy, pp);
-            const VT tq = *reinterpret_cast<const VT *>(Q + base);
-            const VB tc = *reinterpret_cast<const VB *>(p.code + base);
+            if (z0 >= 1) cvt(vload<VT, false>(reinterpret_cast<const VT *>(P + (long long)(z0 - 1) * sxy + cell0)), pm, sP);
+            cvt(vload<VT, false>(reinterpret_cast<const VT *>(P + (long long)z0 * sxy + cell0)), pc, sP);
+        }
+        auto issue = [&](int z) {
+            const long long base = (long long)z * sxy + cell0;
+            if (active) {
+                if (z + 1 < nz) tpp = vload<VT, false>(reinterpret_cast<const VT *>(P + base + sxy));
+                tq = vload<VT, true>(reinterpret_cast<const VT *>(Q + base));
+                tcd = vload<VB, true>(reinterpret_cast<const VB *>(p.code + base));
+            }
+            if (has_yn) tyn = vload<VT, false>(reinterpret_cast<const VT *>(P + base + nx));
+            if (has_ys) tys = vload<VT, false>(reinterpret_cast<const VT *>(P + base - nx));
+            if (need_l) tl = P[base - 1];
+            if (need_r) tr = P[base + V];
+        };
+        issue(z0);
+
+        for (int z = z0; z < z1; ++z) {
+            const long long base = (long long)z * sxy + cell0;
+            C pp[V], qv[V], yn[V], ys[V];
+            uint8_t cd[V];
+            cvt(tpp, pp, sP);
+            if (!(active && z + 1 < nz))
+#pragma unroll
+                for (int j = 0; j < V; ++j) pp[j] = (C)0;
+            cvt(tq, qv, sQ);
+            cvt(tyn, yn, sP);
+            cvt(tys, ys, sP);
+#pragma unroll
+            for (int j = 0; j < V; ++j) cd[j] = active ? tcd.v[j] : (uint8_t)(kTypeZero << 6);
+            C left = __shfl_up(pc[V - 1], 1);
+            C right = __shfl_down(pc[0], 1);
+            if (need_l) left = S::kScaled ? S::ld(tl) * sP : S::ld(tl);
+            if (need_r) right = S::kScaled ? S::ld(tr) * sP : S::ld(tr);
+            if (z + 1 < z1) issue(z + 1);
+
+            VT out;
+            C o[V];
 #pragma unroll
             for (int j = 0; j < V; ++j) {
-                if constexpr (S::kScaled) qv[j] = S::ld(tq.v[j]) * sQ;
-                else qv[j] = S::ld(tq.v[j]);
-                cd[j] = tc.v[j];
-            }
-            if (y + 1 < p.ny) loadP(base + nx, yn);
-            if (y > 0) loadP(base - nx, ys);
-        }
-        C left = __shfl_up(pc[V - 1], 1);
-        C right = __shfl_down(pc[0], 1);
-        if (active && lane == 0 && x0 > 0) {
-            if constexpr (S::kScaled) left = S::ld(P[base - 1]) * sP;
-            else left = S::ld(P[base - 1]);
-        }
-        if (active && lane == 63 && x0 + V < nx) {
-            if constexpr (S::kScaled) right = S::ld(P[base + V]) * sP;
-            else right = S::ld(P[base + V]);
-        }
-        VT out;
-#pragma unroll
-        for (int j = 0; j < V; ++j) {
-            const uint8_t c = cd[j];
-            const int type = c >> 6;
-            C o = (C)0;
-            if (type == kTypeUpdate) {
-                const C pcv = pc[j];
-                const C xp = (c & 1) ? pcv : (j + 1 < V ? pc[(j + 1) % V] : right);
-                const C xm = (c & 2) ? pcv : (j > 0 ? pc[(j + V - 1) % V] : left);
-                const C yp = (c & 4) ? pcv : yn[j];
-                const C ym = (c & 8) ? pcv : ys[j];
-                const C zp = (c & 16) ? pcv : pp[j];
-                const C zm = (c & 32) ? pcv : pm[j];
-                C sum = xp + xm;
-                sum = sum + yp;
-                sum = sum + ym;
-                sum = sum + zp;
-                sum = sum + zm;
-                const C lap = (sum - (C)6 * pcv) * ih2;
-                C a = (C)2, b = (C)1, g = (C)1;
-                if (p.sp.w > 0) {
-                    const int d = edge_dist(x0 + j, y, z, nx, p.ny, p.nz);
-                    if (d < p.sp.w) {
-                        a = (C)p.sp.a[d];
-                        b = (C)p.sp.b[d];
-                        g = (C)p.sp.g[d];
+                const uint8_t c = cd[j];
+                const int type = c >> 6;
+                C ov = (C)0;
+                if (type == kTypeUpdate || type == kTypeBoundary) {
+                    const C pcv = pc[j];
+                    const C xp = (c & 1) ? pcv : (j + 1 < V ? pc[(j + 1) % V] : right);
+                    const C xm = (c & 2) ? pcv : (j > 0 ? pc[(j + V - 1) % V] : left);
+                    const C yp = (c & 4) ? pcv : yn[j];
+                    const C ym = (c & 8) ? pcv : ys[j];
+                    const C zp = (c & 16) ? pcv : pp[j];
+                    const C zm = (c & 32) ? pcv : pm[j];
+                    C sum = xp + xm;
+                    sum = sum + yp;
+                    sum = sum + ym;
+                    sum = sum + zp;
+                    sum = sum + zm;
+                    const C lap = (sum - (C)6 * pcv) * ih2;
+                    C a = (C)2, b = (C)1, g = (C)1;
+                    if (sponge) {
+                        const int d = edge_dist(x0 + j, y, z, nx, ny, nz);
+                        if (d < p.sp.w) {
+                            a = (C)sp_tab[0][d];
+                            b = (C)sp_tab[1][d];
+                            g = (C)sp_tab[2][d];
+                        }
                     }
+                    ov = (a * pcv - b * qv[j]) + g * (c2dt2 * lap);
+                } else if (type == kTypeHard) {
+                    ov = hard;
                 }
-                o = (a * pcv - b * qv[j]) + g * (c2dt2 * lap);
-            } else if (type == kTypeHard) {
-                o = hard;
+                o[j] = ov;
             }
-            if constexpr (S::kScaled) {
-                lmax = fmaxf(lmax, fabsf((float)o));
-                out.v[j] = S::st(o * sR);
-            } else {
-                out.v[j] = S::st(o);
-            }
-        }
-        if (active) *reinterpret_cast<VT *>(R + base) = out;
+
+            if (fused) {
+                const long long t = (long long)z * p.nchunks_plane + pchunk;
+                const long long eb = p.bstart[t], ee = p.bstart[t + 1];
+                if (eb < ee) { // uniform over the block
+                    int cnt = 0;
 #pragma unroll
-        for (int j = 0; j < V; ++j) {
-            pm[j] = pc[j];
-            pc[j] = pp[j];
+                    for (int j = 0; j < V; ++j) cnt += (cd[j] >> 6) == kTypeBoundary;
+                    // exclusive prefix of cnt over the block, lane order = cell order
+                    const unsigned long long lt = (1ull << lane) - 1ull;
+                    int pre = 0, tot = 0;
+#pragma unroll
+                    for (int bit = 0; bit < 4; ++bit) {
+                        const unsigned long long m = __ballot((cnt >> bit) & 1);
+                        pre += __popcll(m & lt) << bit;
+                        tot += __popcll(m) << bit;
+                    }
+                    if (lane == 0) wsum[bparity][wave] = tot;
+                    __syncthreads();
+                    for (int k = 0; k < wave; ++k) pre += wsum[bparity][k];
+                    bparity ^= 1;
+                    long long e = eb + pre;
+#pragma unroll
+                    for (int j = 0; j < V; ++j)
+                        if ((cd[j] >> 6) == kTypeBoundary) {
+                            o[j] = boundary_update<C>(p.bs, e, n, pc[j] - qv[j], o[j]);
+                            ++e;
+                        }
+                }
+            }
+
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                if constexpr (S::kScaled) {
+                    lmax = fmaxf(lmax, fabsf((float)o[j]));
+                    out.v[j] = S::st(o[j] * sR);
+                } else {
+                    out.v[j] = S::st(o[j]);
+                }
+            }
+            if (active) vstore<VT, true>(reinterpret_cast<VT *>(R + base), out);
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                pm[j] = pc[j];
+                pc[j] = pp[j];
+            }
         }
     }
 
     if constexpr (S::kScaled) {
-        __shared__ float red[kBlock / 64];
+        __shared__ float red[kWaves];
         lmax = wave_max(lmax);
-        if (lane == 0) red[threadIdx.x >> 6] = lmax;
+        if (lane == 0) red[wave] = lmax;
         __syncthreads();
         if (threadIdx.x == 0) {
             float m = red[0];
-            for (int w = 1; w < kBlock / 64; ++w) m = fmaxf(m, red[w]);
+            for (int k = 1; k < kWaves; ++k) m = fmaxf(m, red[k]);
             atomicMax(&p.ctl->maxabs[n & 3][bid % kSlots], __float_as_uint(m));
         }
     }
@@ -359,14 +527,10 @@ struct TailParams {
     const void *P;
     const void *Q;
     void *R;
-    long long nb;
-    const uint32_t *bidx;
-    const uint8_t *bmat;     // may be null: row 0
-    double *s0, *s1, *s2, *s3; // forward: x1 x2 y1 y2; reverse: ybar1 ybar2 dbar1 dbar2
-    const double *coeffs;    // [nmat][5]
-    int nmat;
-    float kconst;            // (c*dt)/h*0.95f
+    const uint32_t *bidx;    // compact list; bs.nb = 0 when the stencil fused it
+    BoundaryState bs;
     StepCtl *ctl;
+    int slot, next_slot;     // reads nring[slot], writes nring[next_slot]
     int nsoft;
     const long long *soft_idx, *soft_bpos;
     const double *soft_sig;
@@ -376,8 +540,6 @@ struct TailParams {
     double *ir;              // forward: [nrcv][ir_cap] output
     const double *gsig;      // reverse: dL/dIR [nrcv][ir_cap]
     long long ir_cap;
-    double *trace_d, *trace_y; // [slots][nb]; slot = n - trace_base + 2
-    long long trace_base, trace_slots;
     double *grad_part;       // reverse: [gridDim][nmat][5]
     int nx, ny, nz;
     Sponge sp;
@@ -398,33 +560,16 @@ __device__ inline void sponge_g(const TailParams &p, long long idx, C &g)
     }
 }
 
-__device__ inline void advance_step(StepCtl *ctl, long long n_next)
-{
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned prev = atomicAdd(&ctl->done, 1u);
-        if (prev == gridDim.x - 1) {
-            atomicExch(&ctl->done, 0u);
-            atomicExch((unsigned long long *)&ctl->n, (unsigned long long)n_next);
-            __threadfence();
-        }
-    }
-}
-
-// Boundary biquad over the compact list (simulator_gpu.cu:105-150), in order:
-//   d = P[b] - Q[b]
-//   y = b0 d + b1 x1 + b2 x2 - a1 y1 - a2 y2          (FP64 state)
-//   R[b] = R[b] - K (0.1 y + 0.9 d) 0.4f               (evaluated in double)
-// plus soft sources (added to R before the correction), receiver gather
-// (after it) and the step counter.  The launch is ordered after k_dense on
-// the stream, as the reference's two launches are.
+// The rest of a step after k_dense: the boundary biquad over the compact
+// list when it was not fused into the stencil, then soft sources (added to
+// the corrected field), receiver gather, and the step counter.  Point cells
+// on the list are handled by their entry's thread, the others by block 0.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
 {
     using S = Store<T>;
     using C = typename S::C;
-    const long long n = p.ctl->n;
+    const long long n = p.ctl->nring[p.slot];
     C sP = (C)1, sQ = (C)1, sR = (C)1, sRi = (C)1;
     if constexpr (S::kScaled) {
         sP = exp2i(-p.ctl->exp[(n - 1) & 3]);
@@ -441,45 +586,25 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
     float lmax = 0.f;
 
     const long long e = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (e < p.nb) {
+    if (e < p.bs.nb) {
         const long long b = p.bidx[e];
-        const int m = p.bmat ? p.bmat[e] : 0;
-        const double *cf = p.coeffs + 5 * m;
-        C pv = S::ld(P[b]), qv = S::ld(Q[b]);
-        if constexpr (S::kScaled) pv = pv * sP, qv = qv * sQ;
-        const C dif = pv - qv;
-        const double in = (double)dif;
-        const double x1 = p.s0[e], x2 = p.s1[e], y1 = p.s2[e], y2 = p.s3[e];
-        const double yv = cf[0] * in + cf[1] * x1 + cf[2] * x2 - cf[3] * y1 - cf[4] * y2;
-        p.s1[e] = x1;
-        p.s0[e] = in;
-        p.s3[e] = y1;
-        p.s2[e] = yv;
-        C r = S::ld(R[b]);
-        if constexpr (S::kScaled) r = r * sRi;
+        C pv = S::ld(P[b]), qv = S::ld(Q[b]), r = S::ld(R[b]);
+        if constexpr (S::kScaled) pv = pv * sP, qv = qv * sQ, r = r * sRi;
+        r = boundary_update<C>(p.bs, e, n, pv - qv, r);
         for (int k = 0; k < p.nsoft; ++k)
             if (p.soft_bpos[k] == e && in_sig) r = r + (C)p.soft_sig[(long long)k * p.soft_len + n];
-        r = (C)((double)r - (double)p.kconst * (yv * 0.1 + in * 0.9) * (double)0.4f);
+        C stored = r;
         if constexpr (S::kScaled) {
-            R[b] = S::st(r * sR);
+            const T t = S::st(r * sR);
+            R[b] = t;
+            stored = S::ld(t) * sRi;
             lmax = fabsf((float)r);
         } else {
             R[b] = S::st(r);
         }
-        if (p.trace_d) {
-            const long long slot = n - p.trace_base + 2;
-            if (slot >= 0 && slot < p.trace_slots) {
-                p.trace_d[slot * p.nb + e] = in;
-                p.trace_y[slot * p.nb + e] = yv;
-            }
-        }
         if (in_ir)
             for (int k = 0; k < p.nrcv; ++k)
-                if (p.rcv_bpos[k] == e) {
-                    C rv = r;
-                    if constexpr (S::kScaled) rv = S::ld(S::st(r * sR)) * sRi; // the stored value
-                    p.ir[(long long)k * p.ir_cap + n] = (double)rv;
-                }
+                if (p.rcv_bpos[k] == e) p.ir[(long long)k * p.ir_cap + n] = (double)stored;
     }
     if (blockIdx.x == 0) {
         const int k = threadIdx.x;
@@ -501,19 +626,19 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
             if constexpr (S::kScaled) r = r * sRi;
             p.ir[(long long)k * p.ir_cap + n] = (double)r;
         }
+        if (k == 0) p.ctl->nring[p.next_slot] = n + 1;
     }
     if constexpr (S::kScaled) {
-        __shared__ float red[kBlock / 64];
+        __shared__ float red[kWaves];
         lmax = wave_max(lmax);
         if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = lmax;
         __syncthreads();
         if (threadIdx.x == 0) {
             float mx = red[0];
-            for (int w = 1; w < kBlock / 64; ++w) mx = fmaxf(mx, red[w]);
+            for (int w = 1; w < kWaves; ++w) mx = fmaxf(mx, red[w]);
             atomicMax(&p.ctl->maxabs[n & 3][blockIdx.x % kSlots], __float_as_uint(mx));
         }
     }
-    advance_step(p.ctl, n + 1);
 }
 
 // Reverse boundary step.  With w = g*rbar the adjoint of the field after
@@ -529,44 +654,45 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
 {
     using C = T;
-    __shared__ double part[kBlock / 64][8 * 5];
-    const long long n = p.ctl->n;
+    __shared__ double part[kWaves][8 * 5];
+    const long long n = p.ctl->nring[p.slot];
     const bool in_g = n >= 0 && n < p.ir_cap;
     T *__restrict__ R = (T *)p.R;
+    const BoundaryState &bs = p.bs;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long long e = (long long)blockIdx.x * kBlock + threadIdx.x;
     double gv[5] = {0, 0, 0, 0, 0};
     int m = -1;
-    if (e < p.nb) {
+    if (e < bs.nb) {
         const long long b = p.bidx[e];
-        m = p.bmat ? p.bmat[e] : 0;
-        const double *cf = p.coeffs + 5 * m;
+        m = bs.bmat ? bs.bmat[e] : 0;
+        const double *cf = bs.coeffs + 5 * m;
         double g = 1.0;
         sponge_g(p, b, g);
         double rb = (double)R[b] / g;
-        const double yb1 = p.s0[e], yb2 = p.s1[e], db1 = p.s2[e], db2 = p.s3[e];
+        const double yb1 = bs.s0[e], yb2 = bs.s1[e], db1 = bs.s2[e], db2 = bs.s3[e];
         rb = rb + db1 - db2;
         if (in_g)
             for (int k = 0; k < p.nrcv; ++k)
                 if (p.rcv_bpos[k] == e) rb += p.gsig[(long long)k * p.ir_cap + n];
-        const double kap = (double)p.kconst * (double)0.4f;
+        const double kap = (double)bs.kconst * (double)0.4f;
         const double yb = -0.1 * kap * rb - cf[3] * yb1 - cf[4] * yb2;
         const double db = -0.9 * kap * rb + cf[0] * yb + cf[1] * yb1 + cf[2] * yb2;
-        const long long sl = n - p.trace_base + 2;
-        const double dn = p.trace_d[sl * p.nb + e];
-        const double x1 = p.trace_d[(sl - 1) * p.nb + e];
-        const double x2 = p.trace_d[(sl - 2) * p.nb + e];
-        const double y1 = p.trace_y[(sl - 1) * p.nb + e];
-        const double y2 = p.trace_y[(sl - 2) * p.nb + e];
+        const long long sl = n - bs.trace_base + 2;
+        const double dn = bs.trace_d[sl * bs.nb + e];
+        const double x1 = bs.trace_d[(sl - 1) * bs.nb + e];
+        const double x2 = bs.trace_d[(sl - 2) * bs.nb + e];
+        const double y1 = bs.trace_y[(sl - 1) * bs.nb + e];
+        const double y2 = bs.trace_y[(sl - 2) * bs.nb + e];
         gv[0] = yb * dn;
         gv[1] = yb * x1;
         gv[2] = yb * x2;
         gv[3] = -yb * y1;
         gv[4] = -yb * y2;
-        p.s1[e] = yb1;
-        p.s0[e] = yb;
-        p.s3[e] = db1;
-        p.s2[e] = db;
+        bs.s1[e] = yb1;
+        bs.s0[e] = yb;
+        bs.s3[e] = db1;
+        bs.s2[e] = db;
         R[b] = (C)(rb * g);
     }
     if (blockIdx.x == 0) {
@@ -577,9 +703,10 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
             sponge_g(p, i, g);
             R[i] = (C)((double)R[i] + g * p.gsig[(long long)k * p.ir_cap + n]);
         }
+        if (k == 0) p.ctl->nring[p.next_slot] = n - 1;
     }
     // per-material block sums, fixed reduction order
-    for (int mm = 0; mm < p.nmat; ++mm) {
+    for (int mm = 0; mm < bs.nmat; ++mm) {
 #pragma unroll
         for (int c = 0; c < 5; ++c) {
             const double s = wave_sum(m == mm ? gv[c] : 0.0);
@@ -587,13 +714,12 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
         }
     }
     __syncthreads();
-    if (threadIdx.x < 5 * p.nmat) {
+    if (threadIdx.x < 5 * bs.nmat) {
         double s = part[0][threadIdx.x];
-        for (int w = 1; w < kBlock / 64; ++w) s = s + part[w][threadIdx.x];
-        double *dst = p.grad_part + (long long)blockIdx.x * 5 * p.nmat + threadIdx.x;
+        for (int w = 1; w < kWaves; ++w) s = s + part[w][threadIdx.x];
+        double *dst = p.grad_part + (long long)blockIdx.x * 5 * bs.nmat + threadIdx.x;
         *dst = *dst + s;
     }
-    advance_step(p.ctl, n - 1);
 }
 
 // Converts a stored field to double in physical units (value * 2^-e).
@@ -618,9 +744,9 @@ __global__ void k_from_f64(const double *__restrict__ in, T *__restrict__ out, l
         out[i] = S::st((C)in[i] * s);
 }
 
-// Sets the step counter (and, with reset, the whole control block) without a
-// memset, so it can sit inside a captured graph.
-__global__ void k_ctl_set(StepCtl *ctl, long long n, int reset)
+// Sets the step index of rotation phase `slot` (and, with reset, clears the
+// control block) without a memset, so it can sit inside a captured graph.
+__global__ void k_ctl_set(StepCtl *ctl, int slot, long long n, int reset)
 {
     const int t = threadIdx.x;
     if (reset) {
@@ -628,19 +754,9 @@ __global__ void k_ctl_set(StepCtl *ctl, long long n, int reset)
             ctl->exp[t] = 0;
             for (int s = 0; s < kSlots; ++s) ctl->maxabs[t][s] = 0u;
         }
-        if (t == 0) {
-            ctl->done = 0u;
-            ctl->err = 0;
-        }
+        if (t == 0) ctl->err = 0;
     }
-    if (t == 0) ctl->n = n;
-}
-
-__global__ void k_fill_f64(double *p, long long n, double v)
-{
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x)
-        p[i] = v;
+    if (t == 0) ctl->nring[slot] = n;
 }
 
 // Copies the filter state at a segment start into trace slots 0 (step s0-2)

@@ -33,6 +33,12 @@ struct virs_ctx {
     uint8_t *code = nullptr;
     uint8_t *normals = nullptr;
     bool have_flags = false;
+    std::vector<uint32_t> h_flag2; // interior flag-2 cells of the uploaded flags
+    int nchunks_plane = 0;         // 256-thread chunks per z-plane of the stencil
+    long long *bstart = nullptr;   // [nz*nchunks_plane + 1] list offsets per (chunk, z)
+    bool fused = false;            // boundary filter runs inside the stencil pass
+    long long *neg = nullptr;      // kMaxPoints x -1: point positions when fused
+    int cus = 0;
 
     long long nb = 0;
     uint32_t *bidx = nullptr;
@@ -169,8 +175,23 @@ int upload_points(virs_ctx *c)
     return VIRS_OK;
 }
 
+BoundaryState boundary_state(const virs_ctx *c, double *const *st)
+{
+    BoundaryState b{};
+    b.nb = c->nb;
+    b.bmat = c->bmat;
+    b.s0 = st[0];
+    b.s1 = st[1];
+    b.s2 = st[2];
+    b.s3 = st[3];
+    b.coeffs = c->coeffs;
+    b.nmat = c->nmat;
+    b.kconst = c->kconst;
+    return b;
+}
+
 DenseParams dense_params(const virs_ctx *c, const void *P, const void *Q, void *R, StepCtl *ctl,
-                         int &grid)
+                         int slot)
 {
     DenseParams p{};
     p.P = P;
@@ -182,21 +203,11 @@ DenseParams dense_params(const virs_ctx *c, const void *P, const void *Q, void *
     p.nz = (int)c->nz;
     p.sxy = c->nx * c->ny;
     p.nvec_plane = p.sxy / c->vw;
-    p.nchunks_plane = (int)((p.nvec_plane + kBlock - 1) / kBlock);
-    int zch = c->d.zchunks;
-    if (zch <= 0) {
-        // enough blocks to fill 256 CUs several times over, rounded to a
-        // multiple of 8 so every XCD gets an equal contiguous share
-        const long long want = 4096;
-        zch = (int)std::max<long long>(1, (want + p.nchunks_plane - 1) / p.nchunks_plane);
-    }
-    zch = (int)std::min<long long>(zch, c->nz);
-    p.zc = (int)((c->nz + zch - 1) / zch);
-    const int nzc = (int)((c->nz + p.zc - 1) / p.zc);
-    grid = p.nchunks_plane * nzc;
+    p.nchunks_plane = c->nchunks_plane;
     p.c2dt2 = c->c2dt2;
     p.ih2 = c->ih2;
     p.ctl = ctl;
+    p.slot = slot;
     p.hard_sig = c->hard_sig;
     p.hard_len = c->hard_len;
     p.hard_ref = c->hard_ref;
@@ -204,26 +215,23 @@ DenseParams dense_params(const virs_ctx *c, const void *P, const void *Q, void *
     p.soft_sig = c->soft_sig;
     p.soft_len = c->soft_len;
     p.sp = c->sp;
+    p.bstart = nullptr;
+    p.bs = boundary_state(c, c->st);
     return p;
 }
 
-TailParams tail_params(const virs_ctx *c, const void *P, const void *Q, void *R, StepCtl *ctl)
+TailParams tail_params(const virs_ctx *c, const void *P, const void *Q, void *R, StepCtl *ctl,
+                       int slot, int next_slot)
 {
     TailParams t{};
     t.P = P;
     t.Q = Q;
     t.R = R;
-    t.nb = c->nb;
     t.bidx = c->bidx;
-    t.bmat = c->bmat;
-    t.s0 = c->st[0];
-    t.s1 = c->st[1];
-    t.s2 = c->st[2];
-    t.s3 = c->st[3];
-    t.coeffs = c->coeffs;
-    t.nmat = c->nmat;
-    t.kconst = c->kconst;
+    t.bs = boundary_state(c, c->st);
     t.ctl = ctl;
+    t.slot = slot;
+    t.next_slot = next_slot;
     t.nsoft = c->nsoft;
     t.soft_idx = c->soft_idx;
     t.soft_bpos = c->soft_bpos;
@@ -241,33 +249,60 @@ TailParams tail_params(const virs_ctx *c, const void *P, const void *Q, void *R,
     return t;
 }
 
-int tail_grid(const virs_ctx *c) { return (int)std::max<long long>(1, (c->nb + kBlock - 1) / kBlock); }
+int tail_grid(long long nb) { return (int)std::max<long long>(1, (nb + kBlock - 1) / kBlock); }
 
-template <typename T, bool ADJ>
-hipError_t launch_dense_t(const virs_ctx *c, const DenseParams &p, int grid)
+// Stencil grid: one block per resident slot of the GPU (occupancy x CUs),
+// rounded to a multiple of 8 XCDs, unless the descriptor fixes it.
+template <typename K>
+int dense_grid(const virs_ctx *c, K kernel)
 {
-    switch (c->vw) {
-    case 8: hipLaunchKernelGGL((k_dense<T, 8, ADJ>), dim3(grid), dim3(kBlock), 0, c->stream, p); break;
-    case 4: hipLaunchKernelGGL((k_dense<T, 4, ADJ>), dim3(grid), dim3(kBlock), 0, c->stream, p); break;
-    case 2: hipLaunchKernelGGL((k_dense<T, 2, ADJ>), dim3(grid), dim3(kBlock), 0, c->stream, p); break;
-    default: hipLaunchKernelGGL((k_dense<T, 1, ADJ>), dim3(grid), dim3(kBlock), 0, c->stream, p); break;
-    }
-    return hipGetLastError();
+    long long W = (long long)c->nchunks_plane * c->nz;
+    if (c->d.dense_blocks > 0) return (int)std::min<long long>(c->d.dense_blocks, W);
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kBlock, 0) != hipSuccess || occ <= 0)
+        occ = 4;
+    long long g = (long long)occ * std::max(c->cus, 1);
+    if (g >= 8) g -= g % 8;
+    return (int)std::max<long long>(1, std::min<long long>(g, W));
 }
 
-hipError_t launch_dense(const virs_ctx *c, const DenseParams &p, int grid, bool adj)
+template <typename T, int V, bool ADJ>
+hipError_t launch_dense_v(const virs_ctx *c, const DenseParams &p)
+{
+    if constexpr (sizeof(T) * V > 16) {
+        return hipErrorInvalidValue;
+    } else {
+        auto k = k_dense<T, V, ADJ>;
+        const int grid = dense_grid(c, k);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, c->stream, p);
+        return hipGetLastError();
+    }
+}
+
+template <typename T, bool ADJ>
+hipError_t launch_dense_t(const virs_ctx *c, const DenseParams &p)
+{
+    switch (c->vw) {
+    case 8: return launch_dense_v<T, 8, ADJ>(c, p);
+    case 4: return launch_dense_v<T, 4, ADJ>(c, p);
+    case 2: return launch_dense_v<T, 2, ADJ>(c, p);
+    default: return launch_dense_v<T, 1, ADJ>(c, p);
+    }
+}
+
+hipError_t launch_dense(const virs_ctx *c, const DenseParams &p, bool adj)
 {
     switch (c->d.precision) {
-    case VIRS_FP16: return launch_dense_t<f16, false>(c, p, grid);
-    case VIRS_BF16: return launch_dense_t<bf16, false>(c, p, grid);
-    case VIRS_FP64: return adj ? launch_dense_t<double, true>(c, p, grid) : launch_dense_t<double, false>(c, p, grid);
-    default: return adj ? launch_dense_t<float, true>(c, p, grid) : launch_dense_t<float, false>(c, p, grid);
+    case VIRS_FP16: return launch_dense_t<f16, false>(c, p);
+    case VIRS_BF16: return launch_dense_t<bf16, false>(c, p);
+    case VIRS_FP64: return adj ? launch_dense_t<double, true>(c, p) : launch_dense_t<double, false>(c, p);
+    default: return adj ? launch_dense_t<float, true>(c, p) : launch_dense_t<float, false>(c, p);
     }
 }
 
 hipError_t launch_tail(const virs_ctx *c, const TailParams &t)
 {
-    const int g = tail_grid(c);
+    const int g = tail_grid(t.bs.nb);
     switch (c->d.precision) {
     case VIRS_FP16: hipLaunchKernelGGL(k_tail<f16>, dim3(g), dim3(kBlock), 0, c->stream, t); break;
     case VIRS_BF16: hipLaunchKernelGGL(k_tail<bf16>, dim3(g), dim3(kBlock), 0, c->stream, t); break;
@@ -279,7 +314,7 @@ hipError_t launch_tail(const virs_ctx *c, const TailParams &t)
 
 hipError_t launch_tail_adj(const virs_ctx *c, const TailParams &t)
 {
-    const int g = tail_grid(c);
+    const int g = tail_grid(t.bs.nb);
     if (c->d.precision == VIRS_FP64)
         hipLaunchKernelGGL(k_tail_adj<double>, dim3(g), dim3(kBlock), 0, c->stream, t);
     else
@@ -292,23 +327,53 @@ hipError_t launch_tail_adj(const virs_ctx *c, const TailParams &t)
 int launch_step(virs_ctx *c, double *trace_d = nullptr, double *trace_y = nullptr,
                 long long trace_base = 0, bool receivers = true, hipEvent_t *ev = nullptr)
 {
-    int grid = 0;
-    DenseParams p = dense_params(c, c->f[c->ip], c->f[c->iq], c->f[c->ir], c->ctl, grid);
-    if (ev) HIPC(c, hipEventRecord(ev[0], c->stream));
-    HIPC(c, launch_dense(c, p, grid, false));
-    if (ev) HIPC(c, hipEventRecord(ev[1], c->stream));
-    TailParams t = tail_params(c, c->f[c->ip], c->f[c->iq], c->f[c->ir], c->ctl);
+    DenseParams p = dense_params(c, c->f[c->ip], c->f[c->iq], c->f[c->ir], c->ctl, c->ip);
+    TailParams t = tail_params(c, c->f[c->ip], c->f[c->iq], c->f[c->ir], c->ctl, c->ip, c->ir);
+    BoundaryState &bs = c->fused ? p.bs : t.bs;
+    bs.trace_d = trace_d;
+    bs.trace_y = trace_y;
+    bs.trace_base = trace_base;
+    bs.trace_slots = c->trace_slots;
+    if (c->fused) {
+        p.bstart = c->bstart;
+        t.bs.nb = 0;
+        t.soft_bpos = c->neg;
+        t.rcv_bpos = c->neg;
+    }
     if (!receivers) t.nrcv = 0;
-    t.trace_d = trace_d;
-    t.trace_y = trace_y;
-    t.trace_base = trace_base;
-    t.trace_slots = c->trace_slots;
+    if (ev) HIPC(c, hipEventRecord(ev[0], c->stream));
+    HIPC(c, launch_dense(c, p, false));
+    if (ev) HIPC(c, hipEventRecord(ev[1], c->stream));
     HIPC(c, launch_tail(c, t));
     if (ev) HIPC(c, hipEventRecord(ev[2], c->stream));
     const int ip = c->ip;
     c->ip = c->ir;
     c->ir = c->iq;
     c->iq = ip;
+    return VIRS_OK;
+}
+
+// Decides whether the boundary filter can run inside the stencil (the list
+// must be exactly the interior flag-2 cells, in order) and builds the
+// per-(chunk, z) offsets into the list.
+int refresh_fusion(virs_ctx *c)
+{
+    dfree(c->bstart);
+    c->fused = false;
+    if (!c->d.fuse_boundary || !c->have_flags || c->N > (long long)UINT32_MAX || c->h_bidx != c->h_flag2)
+        return VIRS_OK;
+    const long long nch = c->nchunks_plane, chunk_cells = (long long)kBlock * c->vw;
+    const long long sxy = c->nx * c->ny;
+    std::vector<long long> cnt((size_t)(c->nz * nch + 1), 0);
+    for (uint32_t idx : c->h_bidx) {
+        const long long z = idx / sxy, r = idx - z * sxy;
+        ++cnt[(size_t)(z * nch + r / chunk_cells) + 1];
+    }
+    for (size_t k = 1; k < cnt.size(); ++k) cnt[k] += cnt[k - 1];
+    int rc = dalloc(c, &c->bstart, sizeof(long long) * cnt.size());
+    if (rc) return rc;
+    HIPC(c, hipMemcpy(c->bstart, cnt.data(), sizeof(long long) * cnt.size(), hipMemcpyHostToDevice));
+    c->fused = true;
     return VIRS_OK;
 }
 
@@ -345,7 +410,7 @@ int reset_run(virs_ctx *c)
     if (rc) return rc;
     for (int k = 0; k < 4; ++k)
         if (c->st[k] && c->nb) HIPC(c, hipMemsetAsync(c->st[k], 0, sizeof(double) * c->nb, c->stream));
-    hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl, 0LL, 1);
+    hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl, 0, 0LL, 1);
     HIPC(c, hipGetLastError());
     c->ip = 0;
     c->iq = 1;
@@ -374,7 +439,8 @@ int virs_desc_default(virs_desc *d)
     d->sponge_sigma = 0.f;
     d->device = 0;
     d->use_graph = 0;
-    d->zchunks = 0;
+    d->dense_blocks = 0;
+    d->fuse_boundary = 1;
     return VIRS_OK;
 }
 
@@ -419,7 +485,9 @@ int virs_create(const virs_desc *d, virs_ctx **out)
         c->sp.b[k] = (float)((1.0 - s) / (1.0 + s));
         c->sp.g[k] = (float)(1.0 / (1.0 + s));
     }
+    c->nchunks_plane = (int)((c->nx * c->ny / c->vw + kBlock - 1) / kBlock);
     hipError_t e = hipSetDevice(d->device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, d->device);
     if (e != hipSuccess) {
         delete c;
         return VIRS_ERR_HIP;
@@ -434,6 +502,8 @@ int virs_create(const virs_desc *d, virs_ctx **out)
     if (!rc) rc = dalloc(c, &c->soft_bpos, sizeof(long long) * kMaxPoints);
     if (!rc) rc = dalloc(c, &c->rcv_idx, sizeof(long long) * kMaxPoints);
     if (!rc) rc = dalloc(c, &c->rcv_bpos, sizeof(long long) * kMaxPoints);
+    if (!rc) rc = dalloc(c, &c->neg, sizeof(long long) * kMaxPoints);
+    if (!rc && hipMemset(c->neg, 0xff, sizeof(long long) * kMaxPoints) != hipSuccess) rc = VIRS_ERR_HIP;
     if (!rc) rc = reset_run(c);
     if (!rc && hipMemsetAsync(c->code, kTypeZero << 6, (size_t)c->N, c->stream) != hipSuccess) rc = VIRS_ERR_HIP;
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = VIRS_ERR_HIP;
@@ -470,6 +540,8 @@ int virs_destroy(virs_ctx *c)
     dfree(c->soft_sig);
     dfree(c->rcv_idx);
     dfree(c->rcv_bpos);
+    dfree(c->neg);
+    dfree(c->bstart);
     dfree(c->ir_buf);
     dfree(c->trace_d);
     dfree(c->trace_y);
@@ -511,8 +583,16 @@ int virs_upload_flags(virs_ctx *c, const uint8_t *flags)
     (void)hipFree(dflags);
     if (e != hipSuccess) return fail(c, VIRS_ERR_HIP, "flag upload: %s", hipGetErrorString(e));
     c->have_flags = true;
+    c->h_flag2.clear();
+    const long long nx = c->nx, ny = c->ny, nz = c->nz;
+    for (long long z = 1; z < nz - 1; ++z)
+        for (long long y = 1; y < ny - 1; ++y)
+            for (long long x = 1; x < nx - 1; ++x) {
+                const long long i = (z * ny + y) * nx + x;
+                if (flags[i] == 2) c->h_flag2.push_back((uint32_t)i);
+            }
     drop_graphs(c);
-    return VIRS_OK;
+    return refresh_fusion(c);
 }
 
 int virs_upload_normals(virs_ctx *c, const uint8_t *normals)
@@ -553,6 +633,8 @@ int virs_upload_boundary(virs_ctx *c, const uint32_t *idx, const uint8_t *mat, i
     for (int k = 0; k < 4; ++k)
         if (nb) HIPC(c, hipMemset(c->st[k], 0, sizeof(double) * nb));
     drop_graphs(c);
+    rc = refresh_fusion(c);
+    if (rc) return rc;
     return upload_points(c);
 }
 
@@ -793,7 +875,7 @@ int virs_field_exponent(virs_ctx *c, int32_t which, int32_t *e)
     HIPC(c, hipStreamSynchronize(c->stream));
     StepCtl h;
     HIPC(c, hipMemcpy(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost));
-    *e = h.exp[(h.n - 1 - which) & 3];
+    *e = h.exp[(h.nring[c->ip] - 1 - which) & 3];
     return VIRS_OK;
 }
 
@@ -850,7 +932,7 @@ int virs_write_field_f64(virs_ctx *c, int32_t which, const double *in)
         }
         StepCtl h;
         HIPC(c, hipMemcpy(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost));
-        const int slot = (int)((h.n - 1 - which) & 3);
+        const int slot = (int)((h.nring[c->ip] - 1 - which) & 3);
         h.exp[slot] = e;
         const float fm = (float)mx;
         unsigned bits;
@@ -1044,7 +1126,7 @@ int virs_adjoint(virs_ctx *c, const float *dl_dir, double *grad)
         for (long long i = 0; i < (long long)c->nrcv * T; ++i) g[i] = dl_dir[i];
         HIPC(c, hipMemcpy(c->gsig, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice));
     }
-    const long long gb = tail_grid(c);
+    const long long gb = tail_grid(c->nb);
     if (c->grad_blocks != gb || !c->grad_part) {
         dfree(c->grad_part);
         rc = dalloc(c, &c->grad_part, sizeof(double) * 5 * 8 * gb);
@@ -1056,7 +1138,7 @@ int virs_adjoint(virs_ctx *c, const float *dl_dir, double *grad)
     if (rc) return rc;
     for (int k = 0; k < 4; ++k)
         if (c->nb) HIPC(c, hipMemsetAsync(c->ast[k], 0, sizeof(double) * c->nb, c->stream));
-    hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl_adj, T - 1, 1);
+    hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl_adj, 0, T - 1, 1);
     HIPC(c, hipGetLastError());
     int ap = 0, aq = 1, ar = 2;
     for (int k = c->ncp - 1; k >= 0; --k) {
@@ -1071,7 +1153,7 @@ int virs_adjoint(virs_ctx *c, const float *dl_dir, double *grad)
             for (int j = 0; j < 4 && c->nb; ++j)
                 HIPC(c, hipMemcpyAsync(c->st[j], c->cp_state[k] + j * c->nb, sizeof(double) * c->nb,
                                        hipMemcpyDeviceToDevice, c->stream));
-            hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl, s0, 0);
+            hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl, 0, s0, 0);
             HIPC(c, hipGetLastError());
             rc = seed_trace(c);
             if (rc) return rc;
@@ -1082,23 +1164,19 @@ int virs_adjoint(virs_ctx *c, const float *dl_dir, double *grad)
             c->last_seg_base = s0;
         }
         for (long long s = s1 - 1; s >= s0; --s) {
-            int grid = 0;
-            DenseParams p = dense_params(c, c->af[ap], c->af[aq], c->af[ar], c->ctl_adj, grid);
+            DenseParams p = dense_params(c, c->af[ap], c->af[aq], c->af[ar], c->ctl_adj, ap);
             p.nsoft = 0;
-            HIPC(c, launch_dense(c, p, grid, true));
-            TailParams t = tail_params(c, c->af[ap], c->af[aq], c->af[ar], c->ctl_adj);
-            t.s0 = c->ast[0];
-            t.s1 = c->ast[1];
-            t.s2 = c->ast[2];
-            t.s3 = c->ast[3];
+            HIPC(c, launch_dense(c, p, true));
+            TailParams t = tail_params(c, c->af[ap], c->af[aq], c->af[ar], c->ctl_adj, ap, ar);
+            t.bs = boundary_state(c, c->ast);
+            t.bs.trace_d = c->trace_d;
+            t.bs.trace_y = c->trace_y;
+            t.bs.trace_base = s0;
+            t.bs.trace_slots = c->trace_slots;
             t.ir = nullptr;
             t.gsig = c->gsig;
             t.ir_cap = T;
             t.nsoft = 0;
-            t.trace_d = c->trace_d;
-            t.trace_y = c->trace_y;
-            t.trace_base = s0;
-            t.trace_slots = c->trace_slots;
             t.grad_part = c->grad_part;
             HIPC(c, launch_tail_adj(c, t));
             const int tp = ap;
