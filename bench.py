@@ -40,7 +40,7 @@ def step_bytes(n, nb, nrcv, s):
     return n * (3 * s + 1) + nb * (69 + 4 * s) + nrcv * s
 
 
-def run_gpu(prep, precision, steps, warmup, dense_blocks=0, fuse=True):
+def run_gpu(prep, precision, steps, warmup, dense_blocks=0, fuse=False):
     from virs_amd.fdtd import BYTES, setup
     f = setup(prep, precision, dense_blocks=dense_blocks, fuse_boundary=fuse)
     f.start(warmup + steps)
@@ -94,7 +94,7 @@ def main():
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "fp64"])
     ap.add_argument("--variants", default="fp16", help="extra storage precisions to time, comma separated")
     ap.add_argument("--dense-blocks", type=int, default=0, help="stencil grid size (0 = fill the GPU once)")
-    ap.add_argument("--no-fuse", action="store_true", help="run the boundary list as its own launch")
+    ap.add_argument("--fuse", action="store_true", help="run the boundary filter inside the stencil pass")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes 5 times (the reference "
@@ -124,7 +124,7 @@ def main():
     nb = len(prep.room.bidx)
     nrcv = len(prep.rcv_idx)
 
-    r = run_gpu(prep, args.precision, args.steps, args.warmup, args.dense_blocks, not args.no_fuse)
+    r = run_gpu(prep, args.precision, args.steps, args.warmup, args.dense_blocks, args.fuse)
     ms = r["total_ms"] / args.steps
     value = n * args.steps / (r["total_ms"] / 1e3) / 1e9
     s = r["s"]
@@ -159,7 +159,7 @@ def main():
     }
     variants = {}
     for prec in [v for v in args.variants.split(",") if v and v != args.precision]:
-        rv = run_gpu(prep, prec, args.steps, args.warmup, args.dense_blocks, not args.no_fuse)
+        rv = run_gpu(prep, prec, args.steps, args.warmup, args.dense_blocks, args.fuse)
         vv = n * args.steps / (rv["total_ms"] / 1e3) / 1e9
         db = n * (3 * rv["s"] + 1)
         variants[prec] = {"value": round(vv, 3), "speedup_vs_main": round(vv / value, 3),

@@ -41,6 +41,10 @@ extern "C" {
 #define VIRS_BF16 2
 #define VIRS_FP64 3
 
+/* Storage rounding of FP16/BF16 fields (FP32/FP64 always round to nearest). */
+#define VIRS_ROUND_NEAREST 0
+#define VIRS_ROUND_STOCHASTIC 1
+
 #define VIRS_MAX_SPONGE 64
 #define VIRS_MAX_POINTS 64
 
@@ -60,11 +64,12 @@ typedef struct {
     int32_t dense_blocks;   /* stencil grid size; 0 = fill the GPU once */
     int32_t fuse_boundary;  /* 1: run the boundary biquad inside the stencil
                                pass when the list is the grid's interior
-                               flag-2 cells; 0: always a separate list launch */
+                               flag-2 cells; 0: a separate list launch */
+    int32_t rounding;       /* VIRS_ROUND_*: storage rounding of 16-bit fields */
 } virs_desc;
 
 /* Fills the reference defaults: c = 343, dt = 0, FP32, no sponge, device 0,
- * boundary fused into the stencil pass. */
+ * boundary list as its own launch, stochastic rounding of 16-bit fields. */
 int virs_desc_default(virs_desc *d);
 
 /* fdtd_gpu_setup (simulator_gpu.cpp:7-235): allocate the device grid, zeroed. */

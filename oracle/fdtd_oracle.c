@@ -376,3 +376,232 @@ int oracle_adjoint_f64(const oracle_desc *d, int64_t nsteps, const double *dL_di
     free(tr_d); free(tr_y); free(ir); free(ub_next); free(ub_cur); free(ub_prev); free(sadj);
     return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * 16-bit storage runner (FP16 or BF16 fields, FP32 arithmetic, FP64 filter
+ * state), restating the kernels' scaled-storage rules exactly:
+ *   - each stored field has a power-of-two exponent e (stored = value*2^e);
+ *   - the exponent of step n's field is 14 - k, est = m*2^k, where
+ *     est = 16*max(M(n-1), M(n-2)) + |hard(n)| + sum_k |soft_k(n)| and M(s)
+ *     is the largest |value| computed for step s before storage rounding;
+ *   - storage rounding is nearest-even or stochastic: the bits below the
+ *     16-bit format's last place are replaced by a counter-based hash of
+ *     (cell, step, salt) before truncation (mix32 below);
+ *   - the boundary list runs after the stencil on the stored field, soft
+ *     sources after the boundary correction.
+ * ------------------------------------------------------------------------ */
+
+static uint32_t mix32(uint64_t idx, int64_t n, uint32_t salt)
+{
+    uint32_t h = (uint32_t)idx * 0x9E3779B1u ^ (uint32_t)(idx >> 32) * 0x7FEB352Du;
+    h += (uint32_t)n * 0x85EBCA77u + salt;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return h;
+}
+
+static uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+/* IEEE binary16 <-> float, round to nearest even. */
+static uint16_t f32_to_f16_rne(float f)
+{
+    const uint32_t u = f2u(f);
+    const uint32_t sign = (u >> 16) & 0x8000u;
+    uint32_t mag = u & 0x7fffffffu;
+    if (mag >= 0x7f800000u) /* inf / nan */
+        return (uint16_t)(sign | 0x7c00u | (mag > 0x7f800000u ? 0x200u | ((mag >> 13) & 0x3ffu) : 0u));
+    if (mag >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u); /* >= 65520 rounds to inf */
+    if (mag >= 0x38800000u) { /* normal half */
+        const uint32_t lsb = (mag >> 13) & 1u;
+        mag += 0xfffu + lsb;
+        return (uint16_t)(sign | (((mag >> 13) - (112u << 10)) & 0x7fffu));
+    }
+    /* subnormal half (or zero): value = m * 2^-24 */
+    if (mag < 0x33000000u) return (uint16_t)sign; /* < 2^-25: rounds to 0 */
+    const uint32_t e = mag >> 23;                 /* 102..112 */
+    const uint32_t m = (mag & 0x7fffffu) | 0x800000u;
+    const uint32_t shift = 126u - e;              /* bits to drop to reach 2^-24 units */
+    uint32_t q = m >> shift;
+    const uint32_t rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1u);
+    if (rem > half || (rem == half && (q & 1u))) ++q;
+    return (uint16_t)(sign | q);
+}
+
+static float f16_to_f32(uint16_t hbits)
+{
+    const uint32_t sign = (uint32_t)(hbits & 0x8000u) << 16;
+    const uint32_t e = (hbits >> 10) & 0x1fu, m = hbits & 0x3ffu;
+    if (e == 0) return u2f(sign) + (sign ? -1.f : 1.f) * ldexpf((float)m, -24);
+    if (e == 31) return u2f(sign | 0x7f800000u | (m << 13));
+    return u2f(sign | ((e + 112u) << 23) | (m << 13));
+}
+
+static uint16_t f32_to_bf16_rne(float f)
+{
+    uint32_t u = f2u(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+static float bf16_to_f32(uint16_t b) { return u2f((uint32_t)b << 16); }
+
+/* Stochastic rounding: add r (random) to the magnitude's bits below the
+ * target format's last place, then truncate them; the result is exactly
+ * representable and converts without further rounding. */
+static uint16_t f32_to_f16_sr(float f, uint32_t r)
+{
+    const uint32_t u = f2u(f), sign = u & 0x80000000u, mag = u & 0x7fffffffu;
+    if (mag >= 0x7f800000u) return f32_to_f16_rne(f);
+    const int e = (int)(mag >> 23) - 127;
+    int shift = e >= -14 ? 13 : 13 + (-14 - e);
+    if (shift > 24) shift = 24;
+    const uint32_t mask = (1u << shift) - 1u;
+    const uint32_t mag2 = (mag + (r & mask)) & ~mask;
+    return f32_to_f16_rne(u2f(sign | mag2));
+}
+
+static uint16_t f32_to_bf16_sr(float f, uint32_t r)
+{
+    const uint32_t u = f2u(f), sign = u & 0x80000000u, mag = u & 0x7fffffffu;
+    if (mag >= 0x7f800000u) return f32_to_bf16_rne(f);
+    const uint32_t mag2 = (mag + (r & 0xffffu)) & 0xffff0000u;
+    return (uint16_t)((sign | mag2) >> 16);
+}
+
+/* salts of the three places a cell may be stored within one step */
+#define SALT_DENSE 0u
+#define SALT_BOUNDARY 0x68E31DA4u
+#define SALT_SOURCE 0xB5297A4Du
+
+typedef struct {
+    int bf16;   /* 0 FP16, 1 BF16 */
+    int sr;     /* 1 stochastic rounding */
+} store16;
+
+static float ld16(const store16 *s, uint16_t v) { return s->bf16 ? bf16_to_f32(v) : f16_to_f32(v); }
+
+static uint16_t st16(const store16 *s, float x, uint64_t idx, int64_t n, uint32_t salt)
+{
+    if (s->sr) {
+        const uint32_t r = mix32(idx, n, salt);
+        return s->bf16 ? f32_to_bf16_sr(x, r) : f32_to_f16_sr(x, r);
+    }
+    return s->bf16 ? f32_to_bf16_rne(x) : f32_to_f16_rne(x);
+}
+
+static float exp2i_f(int e) { return u2f((uint32_t)(e + 127) << 23); }
+
+int oracle_run_s16(const oracle_desc *d, int bf16, int sr, int64_t nsteps, double *ir,
+                   double *final_p, double *final_q, int32_t *exps)
+{
+    const int nx = d->nx, ny = d->ny, nz = d->nz;
+    const int64_t sxy = (int64_t)nx * ny, N = sxy * nz;
+    const store16 S = {bf16, sr};
+    uint16_t *P = (uint16_t *)calloc((size_t)N, 2), *Q = (uint16_t *)calloc((size_t)N, 2),
+             *R = (uint16_t *)calloc((size_t)N, 2);
+    double *st = (double *)calloc((size_t)(d->nb > 0 ? d->nb : 1) * 4, sizeof(double));
+    int64_t *bpos = (int64_t *)malloc(sizeof(int64_t) * (size_t)(d->nsoft > 0 ? d->nsoft : 1));
+    if (!P || !Q || !R || !st || !bpos || d->init_p || d->init_q) {
+        free(P); free(Q); free(R); free(st); free(bpos);
+        return -1;
+    }
+    for (int k = 0; k < d->nsoft; ++k) {
+        bpos[k] = -1;
+        for (int64_t e = 0; e < d->nb; ++e)
+            if ((int64_t)d->bidx[e] == d->soft_idx[k]) bpos[k] = e;
+    }
+    int ering[4] = {0, 0, 0, 0};
+    float mring[4] = {0.f, 0.f, 0.f, 0.f};
+    const float c2dt2 = d->c2dt2, ih2 = d->ih2;
+    for (int64_t n = 0; n < nsteps; ++n) {
+        const float hard = (float)hard_value(d, n);
+        const float sP = exp2i_f(-ering[(n - 1) & 3]), sQ = exp2i_f(-ering[(n - 2) & 3]);
+        float est = 16.f * fmaxf(mring[(n - 1) & 3], mring[(n - 2) & 3]) + fabsf(hard);
+        for (int k = 0; k < d->nsoft; ++k)
+            if (n < d->soft_len) est += fabsf((float)d->soft_sig[(int64_t)k * d->soft_len + n]);
+        int eR = 0;
+        if (est > 0.f && est < 3.0e38f) {
+            int k2;
+            frexpf(est, &k2);
+            eR = 14 - k2;
+            if (eR < -100) eR = -100;
+            if (eR > 100) eR = 100;
+        }
+        const float sR = exp2i_f(eR), sRi = exp2i_f(-eR);
+        float mx = 0.f;
+        _Pragma("omp parallel for num_threads(g_threads) schedule(static) reduction(max:mx)")
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x) {
+                    const int64_t i = (int64_t)z * sxy + (int64_t)y * nx + x;
+                    float o = 0.f;
+                    const int border = x == 0 || y == 0 || z == 0 || x == nx - 1 || y == ny - 1 || z == nz - 1;
+                    const uint8_t f = d->flags[i];
+                    if (!border && f == 3) {
+                        o = hard;
+                    } else if (!border && (f == 0 || f == 2)) {
+                        const float pc = ld16(&S, P[i]) * sP;
+                        const float v0 = d->flags[i + 1] == 1 ? pc : ld16(&S, P[i + 1]) * sP;
+                        const float v1 = d->flags[i - 1] == 1 ? pc : ld16(&S, P[i - 1]) * sP;
+                        const float v2 = d->flags[i + nx] == 1 ? pc : ld16(&S, P[i + nx]) * sP;
+                        const float v3 = d->flags[i - nx] == 1 ? pc : ld16(&S, P[i - nx]) * sP;
+                        const float v4 = d->flags[i + sxy] == 1 ? pc : ld16(&S, P[i + sxy]) * sP;
+                        const float v5 = d->flags[i - sxy] == 1 ? pc : ld16(&S, P[i - sxy]) * sP;
+                        float sum = v0 + v1;
+                        sum = sum + v2;
+                        sum = sum + v3;
+                        sum = sum + v4;
+                        sum = sum + v5;
+                        const float lap = (sum - 6.f * pc) * ih2;
+                        float a = 2.f, b = 1.f, g = 1.f;
+                        if (d->sponge_w > 0) {
+                            const int dd = sponge_dist(d, x, y, z);
+                            if (dd < d->sponge_w) { a = d->spg_a[dd]; b = d->spg_b[dd]; g = d->spg_g[dd]; }
+                        }
+                        o = (a * pc - b * (ld16(&S, Q[i]) * sQ)) + g * (c2dt2 * lap);
+                    }
+                    if (fabsf(o) > mx) mx = fabsf(o);
+                    R[i] = st16(&S, o * sR, (uint64_t)i, n, SALT_DENSE);
+                }
+        for (int64_t e = 0; e < d->nb; ++e) {
+            const int64_t bi = d->bidx[e];
+            const int m = d->bmat ? d->bmat[e] : 0;
+            const double *c = d->coeffs + 5 * m;
+            const float dif = ld16(&S, P[bi]) * sP - ld16(&S, Q[bi]) * sQ;
+            double *s4 = st + 4 * e;
+            const double in = (double)dif;
+            const double yv = c[0] * in + c[1] * s4[0] + c[2] * s4[1] - c[3] * s4[2] - c[4] * s4[3];
+            s4[1] = s4[0]; s4[0] = in; s4[3] = s4[2]; s4[2] = yv;
+            float r = ld16(&S, R[bi]) * sRi;
+            r = (float)((double)r - (double)d->kconst * (yv * 0.1 + in * 0.9) * (double)0.4f);
+            for (int k = 0; k < d->nsoft; ++k)
+                if (bpos[k] == e && n < d->soft_len) r = r + (float)d->soft_sig[(int64_t)k * d->soft_len + n];
+            if (fabsf(r) > mx) mx = fabsf(r);
+            R[bi] = st16(&S, r * sR, (uint64_t)bi, n, SALT_BOUNDARY);
+        }
+        for (int k = 0; k < d->nsoft; ++k) {
+            if (bpos[k] >= 0 || n >= d->soft_len) continue;
+            const int64_t si = d->soft_idx[k];
+            float r = ld16(&S, R[si]) * sRi;
+            r = r + (float)d->soft_sig[(int64_t)k * d->soft_len + n];
+            if (fabsf(r) > mx) mx = fabsf(r);
+            R[si] = st16(&S, r * sR, (uint64_t)si, n, SALT_SOURCE);
+        }
+        ering[n & 3] = eR;
+        mring[n & 3] = mx;
+        if (exps) exps[n] = eR;
+        uint16_t *t = Q; Q = P; P = R; R = t;
+        for (int r = 0; r < d->nrcv; ++r)
+            ir[(int64_t)r * nsteps + n] = (double)(ld16(&S, P[d->rcv_idx[r]]) * sRi);
+    }
+    const float eP = exp2i_f(-ering[(nsteps - 1) & 3]), eQ = exp2i_f(-ering[(nsteps - 2) & 3]);
+    if (final_p) for (int64_t i = 0; i < N; ++i) final_p[i] = (double)(ld16(&S, P[i]) * eP);
+    if (final_q) for (int64_t i = 0; i < N; ++i) final_q[i] = (double)(ld16(&S, Q[i]) * eQ);
+    free(P); free(Q); free(R); free(st); free(bpos);
+    return 0;
+}

@@ -58,6 +58,9 @@ def lib():
         L.oracle_refcpu_steps.restype = None
         L.oracle_refcpu_steps.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_float,
                                           C.c_int32, fpp, fpp, fpp]
+        L.oracle_run_s16.restype = C.c_int
+        L.oracle_run_s16.argtypes = [C.POINTER(OracleDesc), C.c_int, C.c_int, C.c_int64, dp, dp, dp,
+                                     C.POINTER(C.c_int32)]
         L.oracle_set_threads.restype = None
         L.oracle_set_threads.argtypes = [C.c_int]
         _lib = L
@@ -175,6 +178,28 @@ class Problem:
             out["q"] = fq.reshape(self.nz, self.ny, self.nx)
         if want_trace:
             out["trace_d"], out["trace_y"] = td, ty
+        return out
+
+    def run16(self, nsteps, fmt="fp16", stochastic=True, want_fields=False):
+        """16-bit storage run (scaled FP16 or BF16 fields, FP32 arithmetic),
+        the kernels' storage rules restated; from zero fields."""
+        L = lib()
+        d = self.desc()
+        ir = np.zeros((max(1, len(self.rcv_idx)), nsteps), np.float64)
+        N = self.nx * self.ny * self.nz
+        fp = np.zeros(N) if want_fields else None
+        fq = np.zeros(N) if want_fields else None
+        ex = np.zeros(nsteps, np.int32)
+        dp = C.POINTER(C.c_double)
+        ptr = lambda a: None if a is None else a.ctypes.data_as(dp)
+        rc = L.oracle_run_s16(C.byref(d), int(fmt == "bf16"), int(bool(stochastic)), nsteps, ptr(ir),
+                              ptr(fp), ptr(fq), ex.ctypes.data_as(C.POINTER(C.c_int32)))
+        if rc != 0:
+            raise RuntimeError(f"oracle 16-bit run failed: {rc}")
+        out = {"ir": ir[: len(self.rcv_idx)], "exp": ex}
+        if want_fields:
+            out["p"] = fp.reshape(self.nz, self.ny, self.nx)
+            out["q"] = fq.reshape(self.nz, self.ny, self.nx)
         return out
 
     def adjoint(self, nsteps, dl_dir):

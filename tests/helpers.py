@@ -53,11 +53,11 @@ def oracle_problem(p, state_f32=False, with_sponge=True):
                      sponge_width=p["sponge"] if with_sponge else 0, sponge_sigma=p["sigma"])
 
 
-def gpu_context(p, precision="fp32", use_graph=False, dense_blocks=0, fuse=True):
+def gpu_context(p, precision="fp32", use_graph=False, dense_blocks=0, fuse=False, rounding="stochastic"):
     from virs_amd.fdtd import FDTD
     f = FDTD(p["dims"], p["h"], dt=p["dt"], precision=precision, sponge_width=p["sponge"],
              sponge_sigma=p["sigma"], use_graph=use_graph, dense_blocks=dense_blocks,
-             fuse_boundary=fuse)
+             fuse_boundary=fuse, rounding=rounding)
     f.upload_flags(p["flags"])
     f.upload_boundary(p["bidx"], p["bmat"])
     f.set_materials(p["coeffs"])

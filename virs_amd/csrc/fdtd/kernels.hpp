@@ -87,6 +87,53 @@ struct Store<bf16> {
     }
 };
 
+// Stochastic rounding to a 16-bit format: a counter-based hash of
+// (cell, step, salt) supplies the bits below the target's last place, which
+// are added to the magnitude and then truncated, so E[stored] = value and
+// increments smaller than half an ulp are not lost step after step.  The
+// result is exactly representable, so the final conversion is exact.
+__device__ inline uint32_t mix32(unsigned long long idx, long long n, uint32_t salt)
+{
+    uint32_t h = (uint32_t)idx * 0x9E3779B1u ^ (uint32_t)(idx >> 32) * 0x7FEB352Du;
+    h += (uint32_t)n * 0x85EBCA77u + salt;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return h;
+}
+
+constexpr uint32_t kSaltDense = 0u, kSaltBoundary = 0x68E31DA4u, kSaltSource = 0xB5297A4Du;
+
+template <typename T>
+__device__ inline T st_sr(float x, uint32_t r)
+{
+    const uint32_t u = __float_as_uint(x), sign = u & 0x80000000u, mag = u & 0x7fffffffu;
+    if (mag >= 0x7f800000u) return Store<T>::st(x);
+    uint32_t mask;
+    if constexpr (sizeof(T) == 2 && Store<T>::kScaled && !__is_same(T, bf16)) {
+        const int e = (int)(mag >> 23) - 127;
+        const int shift = min(e >= -14 ? 13 : 13 + (-14 - e), 24);
+        mask = (1u << shift) - 1u;
+    } else {
+        mask = 0xffffu;
+    }
+    return Store<T>::st(__uint_as_float(sign | ((mag + (r & mask)) & ~mask)));
+}
+
+// Storage rounding of one value: nearest-even (FP32/FP64 always) or, for
+// the 16-bit formats when enabled, stochastic.
+template <typename T>
+__device__ inline T store_round(typename Store<T>::C x, bool sr, unsigned long long idx, long long n,
+                                uint32_t salt)
+{
+    if constexpr (Store<T>::kScaled) {
+        if (sr) return st_sr<T>(x, mix32(idx, n, salt));
+    }
+    return Store<T>::st(x);
+}
+
 template <typename T, int V>
 struct alignas(sizeof(T) * V) Vec {
     T v[V];
@@ -228,6 +275,7 @@ struct DenseParams {
     long long sxy;
     long long nvec_plane; // vectors of V cells per z-plane
     int nchunks_plane;    // 256-thread chunks per z-plane
+    int zranges;          // > 0: tile mode, block = (chunk, one of zranges z-ranges)
     float c2dt2, ih2;
     StepCtl *ctl;
     int slot;             // rotation phase: reads ctl->nring[slot]
@@ -238,6 +286,7 @@ struct DenseParams {
     const double *soft_sig;
     long long soft_len;
     Sponge sp;
+    int sr;               // stochastic storage rounding (16-bit formats)
     // fused boundary filter: entries of (chunk, z) are
     // [bstart[z*nchunks_plane + chunk], bstart[... + 1]) of the compact list
     const long long *bstart; // null: the boundary list runs in k_tail
@@ -352,10 +401,23 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
     int bparity = 0;
     float lmax = 0.f;
 
-    const long long W = (long long)p.nchunks_plane * nz;
-    const long long per = W / nbk, rem = W % nbk;
-    const long long wbeg = (long long)logical * per + min((long long)logical, rem);
-    const long long wend = wbeg + per + ((long long)logical < rem ? 1 : 0);
+    // Work split over the (chunk, z) planes, flattened chunk-major.
+    //  tile mode: block = chunk (logical % nchunks) x z-range (logical / nchunks),
+    //    so the blocks running together are neighbouring chunks marching the
+    //    same z and their y-halo rows meet in L2;
+    //  otherwise: gridDim contiguous ranges of equal length.
+    long long wbeg, wend;
+    if (p.zranges > 0) {
+        const long long ch = logical % (unsigned)p.nchunks_plane, zr = logical / (unsigned)p.nchunks_plane;
+        const long long za = zr * nz / p.zranges, zb = (zr + 1) * nz / p.zranges;
+        wbeg = ch * nz + za;
+        wend = zr < p.zranges ? ch * nz + zb : wbeg;
+    } else {
+        const long long W = (long long)p.nchunks_plane * nz;
+        const long long per = W / nbk, rem = W % nbk;
+        wbeg = (long long)logical * per + min((long long)logical, rem);
+        wend = wbeg + per + ((long long)logical < rem ? 1 : 0);
+    }
 
     auto cvt = [&](const VT &t, C *out, C s) {
 #pragma unroll
@@ -496,7 +558,7 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
             for (int j = 0; j < V; ++j) {
                 if constexpr (S::kScaled) {
                     lmax = fmaxf(lmax, fabsf((float)o[j]));
-                    out.v[j] = S::st(o[j] * sR);
+                    out.v[j] = store_round<T>(o[j] * sR, p.sr, (unsigned long long)(base + j), n, kSaltDense);
                 } else {
                     out.v[j] = S::st(o[j]);
                 }
@@ -543,6 +605,7 @@ struct TailParams {
     double *grad_part;       // reverse: [gridDim][nmat][5]
     int nx, ny, nz;
     Sponge sp;
+    int sr;                  // stochastic storage rounding (16-bit formats)
 };
 
 template <typename C>
@@ -595,7 +658,7 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
             if (p.soft_bpos[k] == e && in_sig) r = r + (C)p.soft_sig[(long long)k * p.soft_len + n];
         C stored = r;
         if constexpr (S::kScaled) {
-            const T t = S::st(r * sR);
+            const T t = store_round<T>(r * sR, p.sr, (unsigned long long)b, n, kSaltBoundary);
             R[b] = t;
             stored = S::ld(t) * sRi;
             lmax = fabsf((float)r);
@@ -614,7 +677,7 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
             if constexpr (S::kScaled) r = r * sRi;
             r = r + (C)p.soft_sig[(long long)k * p.soft_len + n];
             if constexpr (S::kScaled) {
-                R[i] = S::st(r * sR);
+                R[i] = store_round<T>(r * sR, p.sr, (unsigned long long)i, n, kSaltSource);
                 lmax = fmaxf(lmax, fabsf((float)r));
             } else {
                 R[i] = S::st(r);

@@ -215,6 +215,7 @@ DenseParams dense_params(const virs_ctx *c, const void *P, const void *Q, void *
     p.soft_sig = c->soft_sig;
     p.soft_len = c->soft_len;
     p.sp = c->sp;
+    p.sr = c->d.rounding == VIRS_ROUND_STOCHASTIC;
     p.bstart = nullptr;
     p.bs = boundary_state(c, c->st);
     return p;
@@ -246,22 +247,33 @@ TailParams tail_params(const virs_ctx *c, const void *P, const void *Q, void *R,
     t.ny = (int)c->ny;
     t.nz = (int)c->nz;
     t.sp = c->sp;
+    t.sr = c->d.rounding == VIRS_ROUND_STOCHASTIC;
     return t;
 }
 
 int tail_grid(long long nb) { return (int)std::max<long long>(1, (nb + kBlock - 1) / kBlock); }
 
-// Stencil grid: one block per resident slot of the GPU (occupancy x CUs),
-// rounded to a multiple of 8 XCDs, unless the descriptor fixes it.
+// Stencil grid.  capacity = resident blocks of the whole GPU (occupancy x
+// CUs).  Tile mode (one block per chunk x z-range) is used when whole
+// z-ranges of every chunk fill at least 85% of the capacity in one round;
+// otherwise the (chunk, z) planes are split into `capacity` equal ranges.
+// A descriptor value dense_blocks > 0 forces the split mode with that grid.
 template <typename K>
-int dense_grid(const virs_ctx *c, K kernel)
+int dense_grid(const virs_ctx *c, K kernel, int &zranges)
 {
-    long long W = (long long)c->nchunks_plane * c->nz;
+    zranges = 0;
+    const long long W = (long long)c->nchunks_plane * c->nz;
     if (c->d.dense_blocks > 0) return (int)std::min<long long>(c->d.dense_blocks, W);
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kBlock, 0) != hipSuccess || occ <= 0)
         occ = 4;
-    long long g = (long long)occ * std::max(c->cus, 1);
+    const long long cap = (long long)occ * std::max(c->cus, 1);
+    const long long zr = std::min<long long>(cap / std::max(c->nchunks_plane, 1), c->nz);
+    if (zr >= 1 && zr * c->nchunks_plane * 100 >= cap * 85) {
+        zranges = (int)zr;
+        return (int)(zr * c->nchunks_plane);
+    }
+    long long g = cap;
     if (g >= 8) g -= g % 8;
     return (int)std::max<long long>(1, std::min<long long>(g, W));
 }
@@ -273,8 +285,9 @@ hipError_t launch_dense_v(const virs_ctx *c, const DenseParams &p)
         return hipErrorInvalidValue;
     } else {
         auto k = k_dense<T, V, ADJ>;
-        const int grid = dense_grid(c, k);
-        hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, c->stream, p);
+        DenseParams q = p;
+        const int grid = dense_grid(c, k, q.zranges);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, c->stream, q);
         return hipGetLastError();
     }
 }
@@ -440,7 +453,8 @@ int virs_desc_default(virs_desc *d)
     d->device = 0;
     d->use_graph = 0;
     d->dense_blocks = 0;
-    d->fuse_boundary = 1;
+    d->fuse_boundary = 0;
+    d->rounding = VIRS_ROUND_STOCHASTIC;
     return VIRS_OK;
 }
 
