@@ -40,9 +40,9 @@ def step_bytes(n, nb, nrcv, s):
     return n * (3 * s + 1) + nb * (69 + 4 * s) + nrcv * s
 
 
-def run_gpu(prep, precision, steps, warmup, dense_blocks=0, fuse=False):
+def run_gpu(prep, precision, steps, warmup, dense_blocks=0, fuse=False, vw=0):
     from virs_amd.fdtd import BYTES, setup
-    f = setup(prep, precision, dense_blocks=dense_blocks, fuse_boundary=fuse)
+    f = setup(prep, precision, dense_blocks=dense_blocks, fuse_boundary=fuse, vector_width=vw)
     f.start(warmup + steps)
     if warmup:
         f.step(warmup)
@@ -95,6 +95,7 @@ def main():
     ap.add_argument("--variants", default="fp16", help="extra storage precisions to time, comma separated")
     ap.add_argument("--dense-blocks", type=int, default=0, help="stencil grid size (0 = fill the GPU once)")
     ap.add_argument("--fuse", action="store_true", help="run the boundary filter inside the stencil pass")
+    ap.add_argument("--vw16", type=int, default=0, help="vector width of the 16-bit variants (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes 5 times (the reference "
@@ -159,7 +160,8 @@ def main():
     }
     variants = {}
     for prec in [v for v in args.variants.split(",") if v and v != args.precision]:
-        rv = run_gpu(prep, prec, args.steps, args.warmup, args.dense_blocks, args.fuse)
+        rv = run_gpu(prep, prec, args.steps, args.warmup, args.dense_blocks, args.fuse,
+                     args.vw16 if prec in ("fp16", "bf16") else 0)
         vv = n * args.steps / (rv["total_ms"] / 1e3) / 1e9
         db = n * (3 * rv["s"] + 1)
         variants[prec] = {"value": round(vv, 3), "speedup_vs_main": round(vv / value, 3),

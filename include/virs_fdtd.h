@@ -66,6 +66,8 @@ typedef struct {
                                pass when the list is the grid's interior
                                flag-2 cells; 0: a separate list launch */
     int32_t rounding;       /* VIRS_ROUND_*: storage rounding of 16-bit fields */
+    int32_t vector_width;   /* cells per thread along x in the stencil (1, 2, 4, 8
+                               up to 16 bytes of storage); 0 = widest dividing nx */
 } virs_desc;
 
 /* Fills the reference defaults: c = 343, dt = 0, FP32, no sponge, device 0,

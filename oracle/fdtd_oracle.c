@@ -381,26 +381,28 @@ int oracle_adjoint_f64(const oracle_desc *d, int64_t nsteps, const double *dL_di
  * 16-bit storage runner (FP16 or BF16 fields, FP32 arithmetic, FP64 filter
  * state), restating the kernels' scaled-storage rules exactly:
  *   - each stored field has a power-of-two exponent e (stored = value*2^e);
- *   - the exponent of step n's field is 14 - k, est = m*2^k, where
+ *   - the exponent of step n's field is 14 - k (clamped to [-60, 60]),
+ *     est = m*2^k, where
  *     est = 16*max(M(n-1), M(n-2)) + |hard(n)| + sum_k |soft_k(n)| and M(s)
  *     is the largest |value| computed for step s before storage rounding;
- *   - storage rounding is nearest-even or stochastic: the bits below the
- *     16-bit format's last place are replaced by a counter-based hash of
- *     (cell, step, salt) before truncation (mix32 below);
+ *   - storage rounding is nearest-even or stochastic: 16 bits of a
+ *     counter-based hash of (cell, step, salt) are added below the 16-bit
+ *     format's last place before truncation (mix32 below);
  *   - the boundary list runs after the stencil on the stored field, soft
  *     sources after the boundary correction.
  * ------------------------------------------------------------------------ */
 
+/* 16 random bits for cell idx at step n: one 32-bit hash per cell pair,
+ * low half for the even cell, high half for the odd one. */
 static uint32_t mix32(uint64_t idx, int64_t n, uint32_t salt)
 {
-    uint32_t h = (uint32_t)idx * 0x9E3779B1u ^ (uint32_t)(idx >> 32) * 0x7FEB352Du;
-    h += (uint32_t)n * 0x85EBCA77u + salt;
-    h ^= h >> 15;
+    const uint64_t pair = idx & ~(uint64_t)1;
+    uint32_t h = (uint32_t)pair * 0x9E3779B1u + (uint32_t)(pair >> 32) * 0x7FEB352Du +
+                 (uint32_t)n * 0x85EBCA77u + salt;
+    h ^= h >> 16;
     h *= 0x2C1B3C6Du;
-    h ^= h >> 12;
-    h *= 0x297A2D39u;
     h ^= h >> 15;
-    return h;
+    return (idx & 1) ? (h >> 16) : (h & 0xffffu);
 }
 
 static uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
@@ -450,17 +452,15 @@ static uint16_t f32_to_bf16_rne(float f)
 
 static float bf16_to_f32(uint16_t b) { return u2f((uint32_t)b << 16); }
 
-/* Stochastic rounding: add r (random) to the magnitude's bits below the
- * target format's last place, then truncate them; the result is exactly
- * representable and converts without further rounding. */
+/* Stochastic rounding: add r (16 random bits) to the magnitude below the
+ * format's last place -- 13 bits for FP16, 16 for BF16 -- and truncate
+ * them; the result is exactly representable except in FP16's subnormal
+ * range, where the final conversion rounds to nearest even. */
 static uint16_t f32_to_f16_sr(float f, uint32_t r)
 {
     const uint32_t u = f2u(f), sign = u & 0x80000000u, mag = u & 0x7fffffffu;
     if (mag >= 0x7f800000u) return f32_to_f16_rne(f);
-    const int e = (int)(mag >> 23) - 127;
-    int shift = e >= -14 ? 13 : 13 + (-14 - e);
-    if (shift > 24) shift = 24;
-    const uint32_t mask = (1u << shift) - 1u;
+    const uint32_t mask = 0x1fffu;
     const uint32_t mag2 = (mag + (r & mask)) & ~mask;
     return f32_to_f16_rne(u2f(sign | mag2));
 }
@@ -529,8 +529,8 @@ int oracle_run_s16(const oracle_desc *d, int bf16, int sr, int64_t nsteps, doubl
             int k2;
             frexpf(est, &k2);
             eR = 14 - k2;
-            if (eR < -100) eR = -100;
-            if (eR > 100) eR = 100;
+            if (eR < -60) eR = -60;
+            if (eR > 60) eR = 60;
         }
         const float sR = exp2i_f(eR), sRi = exp2i_f(-eR);
         float mx = 0.f;

@@ -31,7 +31,7 @@ class VirsDesc(C.Structure):
         ("h", C.c_float), ("c", C.c_float), ("dt", C.c_float),
         ("precision", C.c_int32), ("sponge_width", C.c_int32), ("sponge_sigma", C.c_float),
         ("device", C.c_int32), ("use_graph", C.c_int32), ("dense_blocks", C.c_int32),
-        ("fuse_boundary", C.c_int32), ("rounding", C.c_int32),
+        ("fuse_boundary", C.c_int32), ("rounding", C.c_int32), ("vector_width", C.c_int32),
     ]
 
 

@@ -87,39 +87,34 @@ struct Store<bf16> {
     }
 };
 
-// Stochastic rounding to a 16-bit format: a counter-based hash of
-// (cell, step, salt) supplies the bits below the target's last place, which
-// are added to the magnitude and then truncated, so E[stored] = value and
-// increments smaller than half an ulp are not lost step after step.  The
-// result is exactly representable, so the final conversion is exact.
+// Stochastic rounding to a 16-bit format.  A counter-based hash of
+// (cell pair, step, salt) gives 16 random bits per cell (one 32-bit hash
+// serves the two cells 2k and 2k+1); they are added to the float's
+// magnitude below the format's last place (13 bits for FP16, 16 for BF16)
+// and that part is then truncated, so E[stored] = value for every
+// normal-range result and increments smaller than half an ulp are not lost
+// step after step.  The truncated value is exactly representable except in
+// FP16's subnormal range, where the final conversion rounds to nearest.
 __device__ inline uint32_t mix32(unsigned long long idx, long long n, uint32_t salt)
 {
-    uint32_t h = (uint32_t)idx * 0x9E3779B1u ^ (uint32_t)(idx >> 32) * 0x7FEB352Du;
-    h += (uint32_t)n * 0x85EBCA77u + salt;
-    h ^= h >> 15;
+    const unsigned long long pair = idx & ~1ull;
+    uint32_t h = (uint32_t)pair * 0x9E3779B1u + (uint32_t)(pair >> 32) * 0x7FEB352Du +
+                 (uint32_t)n * 0x85EBCA77u + salt;
+    h ^= h >> 16;
     h *= 0x2C1B3C6Du;
-    h ^= h >> 12;
-    h *= 0x297A2D39u;
     h ^= h >> 15;
-    return h;
+    return (idx & 1ull) ? (h >> 16) : (h & 0xffffu);
 }
 
 constexpr uint32_t kSaltDense = 0u, kSaltBoundary = 0x68E31DA4u, kSaltSource = 0xB5297A4Du;
 
 template <typename T>
-__device__ inline T st_sr(float x, uint32_t r)
+__device__ inline T st_sr(float x, uint32_t r16)
 {
     const uint32_t u = __float_as_uint(x), sign = u & 0x80000000u, mag = u & 0x7fffffffu;
     if (mag >= 0x7f800000u) return Store<T>::st(x);
-    uint32_t mask;
-    if constexpr (sizeof(T) == 2 && Store<T>::kScaled && !__is_same(T, bf16)) {
-        const int e = (int)(mag >> 23) - 127;
-        const int shift = min(e >= -14 ? 13 : 13 + (-14 - e), 24);
-        mask = (1u << shift) - 1u;
-    } else {
-        mask = 0xffffu;
-    }
-    return Store<T>::st(__uint_as_float(sign | ((mag + (r & mask)) & ~mask)));
+    constexpr uint32_t mask = __is_same(T, bf16) ? 0xffffu : 0x1fffu;
+    return Store<T>::st(__uint_as_float(sign | ((mag + (r16 & mask)) & ~mask)));
 }
 
 // Storage rounding of one value: nearest-even (FP32/FP64 always) or, for
@@ -324,14 +319,17 @@ __device__ inline C boundary_update(const BoundaryState &bs, long long e, long l
 // A thread owns V consecutive x cells of a row of the flattened z-plane and
 // marches up in z: z neighbours stay in registers, x neighbours come from
 // the vector and the adjacent lanes, y neighbours are the rows +-Nx of the
-// same plane (hits in L2 from the neighbouring chunks).  The next plane's
-// loads are issued before the current plane is computed.
-// Work split: the (chunk, z) planes, flattened chunk-major, are divided into
-// gridDim contiguous ranges of equal length, so every block marches ~the
-// same number of planes and the grid fits the GPU in one round.
-// ADJ: the reverse sweep applies the same operator to the adjoint field;
-// hard-source cells hold prescribed values, so their adjoint is zero.
-template <typename T, int V, bool ADJ>
+// same plane (L2 hits: in tile mode the neighbouring chunks march the same
+// z together).  The next plane's loads are issued before the current plane
+// is computed.  Every cell's update is computed and the result selected by
+// cell type, without branches.
+// 16-bit storage: arithmetic runs in p^n's stored units (value * 2^eP);
+// power-of-two scaling commutes exactly with every rounding step, so the
+// results equal a computation in physical units bit for bit.
+// FUSED: the boundary biquad of the plane's list slice runs in the same
+// pass.  ADJ: the reverse sweep applies the same operator to the adjoint
+// field; hard-source cells hold prescribed values, so their adjoint is zero.
+template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE>
 __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
 {
     using S = Store<T>;
@@ -342,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
     const int wave = threadIdx.x >> 6;
 
     // XCD-aware order: the hardware places block b on XCD b % 8; give each
-    // XCD a contiguous run of logical ranges so neighbouring chunks share an L2.
+    // XCD a contiguous run of logical tiles so neighbouring chunks share an L2.
     const unsigned nbk = gridDim.x, bid = blockIdx.x;
     const unsigned logical = (nbk % 8u == 0u) ? (bid % 8u) * (nbk / 8u) + bid / 8u : bid;
 
@@ -353,7 +351,9 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
         else if (n >= 0 && n < p.hard_len) hard = (C)p.hard_sig[n];
     }
 
-    C sP = (C)1, sQ = (C)1, sR = (C)1;
+    // kq: Q -> P units; kout: P units -> R's stored units; kphys: P units -> physical
+    C kq = (C)1, kout = (C)1, kphys = (C)1;
+    C hard_u = hard;
     if constexpr (S::kScaled) {
         const int ip = (int)((n - 1) & 3), iq = (int)((n - 2) & 3), ir = (int)(n & 3);
         const float mp = wave_max(__uint_as_float(p.ctl->maxabs[ip][lane]));
@@ -365,11 +365,13 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
         if (est > 0.f && est < 3.0e38f) {
             int k2;
             frexpf(est, &k2); // est < 2^k2
-            eR = min(max(14 - k2, -100), 100);
+            eR = min(max(14 - k2, -60), 60);
         }
-        sP = exp2i(-p.ctl->exp[ip]);
-        sQ = exp2i(-p.ctl->exp[iq]);
-        sR = exp2i(eR);
+        const int eP = p.ctl->exp[ip], eQ = p.ctl->exp[iq];
+        kq = exp2i(eP - eQ);
+        kout = exp2i(eR - eP);
+        kphys = exp2i(-eP);
+        hard_u = hard * exp2i(eP);
         if (bid == 0) {
             if (threadIdx.x == 0) {
                 p.ctl->exp[ir] = eR;
@@ -385,12 +387,10 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
     const long long sxy = p.sxy;
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const C c2dt2 = (C)p.c2dt2, ih2 = (C)p.ih2;
-    const bool fused = !ADJ && p.bstart != nullptr;
 
-    __shared__ float sp_tab[3][kMaxSponge];
+    __shared__ float sp_tab[SPONGE ? 3 : 1][SPONGE ? kMaxSponge : 1];
     __shared__ int wsum[2][kWaves];
-    const bool sponge = p.sp.w > 0;
-    if (sponge) {
+    if constexpr (SPONGE) {
         for (int k = threadIdx.x; k < p.sp.w; k += kBlock) {
             sp_tab[0][k] = p.sp.a[k];
             sp_tab[1][k] = p.sp.b[k];
@@ -419,12 +419,9 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
         wend = wbeg + per + ((long long)logical < rem ? 1 : 0);
     }
 
-    auto cvt = [&](const VT &t, C *out, C s) {
+    auto cvt = [&](const VT &t, C *out) {
 #pragma unroll
-        for (int j = 0; j < V; ++j) {
-            if constexpr (S::kScaled) out[j] = S::ld(t.v[j]) * s;
-            else out[j] = S::ld(t.v[j]);
-        }
+        for (int j = 0; j < V; ++j) out[j] = S::ld(t.v[j]);
     };
 
     for (long long w = wbeg; w < wend;) {
@@ -442,8 +439,8 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
         const bool need_l = active && lane == 0 && x0 > 0;
         const bool need_r = active && lane == 63 && x0 + V < nx;
 
-        // registers: pm/pc/pp = P at z-1, z, z+1; the t* values are the raw
-        // loads for the plane being computed next
+        // registers: pm/pc = P at z-1, z (P units); t* = raw loads of the
+        // plane computed next
         C pm[V], pc[V];
         VT tpp{}, tq{}, tyn{}, tys{};
         VB tcd{};
@@ -451,8 +448,8 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
 #pragma unroll
         for (int j = 0; j < V; ++j) pm[j] = pc[j] = (C)0;
         if (active) {
-            if (z0 >= 1) cvt(vload<VT, false>(reinterpret_cast<const VT *>(P + (long long)(z0 - 1) * sxy + cell0)), pm, sP);
-            cvt(vload<VT, false>(reinterpret_cast<const VT *>(P + (long long)z0 * sxy + cell0)), pc, sP);
+            if (z0 >= 1) cvt(vload<VT, false>(reinterpret_cast<const VT *>(P + (long long)(z0 - 1) * sxy + cell0)), pm);
+            cvt(vload<VT, false>(reinterpret_cast<const VT *>(P + (long long)z0 * sxy + cell0)), pc);
         }
         auto issue = [&](int z) {
             const long long base = (long long)z * sxy + cell0;
@@ -472,59 +469,55 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
             const long long base = (long long)z * sxy + cell0;
             C pp[V], qv[V], yn[V], ys[V];
             uint8_t cd[V];
-            cvt(tpp, pp, sP);
-            if (!(active && z + 1 < nz))
+            const bool has_zp = active && z + 1 < nz;
 #pragma unroll
-                for (int j = 0; j < V; ++j) pp[j] = (C)0;
-            cvt(tq, qv, sQ);
-            cvt(tyn, yn, sP);
-            cvt(tys, ys, sP);
-#pragma unroll
-            for (int j = 0; j < V; ++j) cd[j] = active ? tcd.v[j] : (uint8_t)(kTypeZero << 6);
+            for (int j = 0; j < V; ++j) {
+                pp[j] = has_zp ? S::ld(tpp.v[j]) : (C)0;
+                if constexpr (S::kScaled) qv[j] = S::ld(tq.v[j]) * kq;
+                else qv[j] = S::ld(tq.v[j]);
+                yn[j] = S::ld(tyn.v[j]);
+                ys[j] = S::ld(tys.v[j]);
+                cd[j] = active ? tcd.v[j] : (uint8_t)(kTypeZero << 6);
+            }
             C left = __shfl_up(pc[V - 1], 1);
             C right = __shfl_down(pc[0], 1);
-            if (need_l) left = S::kScaled ? S::ld(tl) * sP : S::ld(tl);
-            if (need_r) right = S::kScaled ? S::ld(tr) * sP : S::ld(tr);
+            if (need_l) left = S::ld(tl);
+            if (need_r) right = S::ld(tr);
             if (z + 1 < z1) issue(z + 1);
 
-            VT out;
             C o[V];
 #pragma unroll
             for (int j = 0; j < V; ++j) {
-                const uint8_t c = cd[j];
-                const int type = c >> 6;
-                C ov = (C)0;
-                if (type == kTypeUpdate || type == kTypeBoundary) {
-                    const C pcv = pc[j];
-                    const C xp = (c & 1) ? pcv : (j + 1 < V ? pc[(j + 1) % V] : right);
-                    const C xm = (c & 2) ? pcv : (j > 0 ? pc[(j + V - 1) % V] : left);
-                    const C yp = (c & 4) ? pcv : yn[j];
-                    const C ym = (c & 8) ? pcv : ys[j];
-                    const C zp = (c & 16) ? pcv : pp[j];
-                    const C zm = (c & 32) ? pcv : pm[j];
-                    C sum = xp + xm;
-                    sum = sum + yp;
-                    sum = sum + ym;
-                    sum = sum + zp;
-                    sum = sum + zm;
-                    const C lap = (sum - (C)6 * pcv) * ih2;
-                    C a = (C)2, b = (C)1, g = (C)1;
-                    if (sponge) {
-                        const int d = edge_dist(x0 + j, y, z, nx, ny, nz);
-                        if (d < p.sp.w) {
-                            a = (C)sp_tab[0][d];
-                            b = (C)sp_tab[1][d];
-                            g = (C)sp_tab[2][d];
-                        }
+                const uint32_t c = cd[j];
+                const C pcv = pc[j];
+                const C xp = (c & 1u) ? pcv : (j + 1 < V ? pc[(j + 1) % V] : right);
+                const C xm = (c & 2u) ? pcv : (j > 0 ? pc[(j + V - 1) % V] : left);
+                const C yp = (c & 4u) ? pcv : yn[j];
+                const C ym = (c & 8u) ? pcv : ys[j];
+                const C zp = (c & 16u) ? pcv : pp[j];
+                const C zm = (c & 32u) ? pcv : pm[j];
+                C sum = xp + xm;
+                sum = sum + yp;
+                sum = sum + ym;
+                sum = sum + zp;
+                sum = sum + zm;
+                const C lap = (sum - (C)6 * pcv) * ih2;
+                C a = (C)2, b = (C)1, g = (C)1;
+                if constexpr (SPONGE) {
+                    const int d = edge_dist(x0 + j, y, z, nx, ny, nz);
+                    if (d < p.sp.w) {
+                        a = (C)sp_tab[0][d];
+                        b = (C)sp_tab[1][d];
+                        g = (C)sp_tab[2][d];
                     }
-                    ov = (a * pcv - b * qv[j]) + g * (c2dt2 * lap);
-                } else if (type == kTypeHard) {
-                    ov = hard;
                 }
-                o[j] = ov;
+                const C upd = (a * pcv - b * qv[j]) + g * (c2dt2 * lap);
+                const uint32_t type = c >> 6;
+                // update (0) or update + boundary filter (3); hard source (2); zero (1)
+                o[j] = ((type + 1u) & 2u) == 0u ? upd : (type == kTypeHard ? hard_u : (C)0);
             }
 
-            if (fused) {
+            if constexpr (FUSED) {
                 const long long t = (long long)z * p.nchunks_plane + pchunk;
                 const long long eb = p.bstart[t], ee = p.bstart[t + 1];
                 if (eb < ee) { // uniform over the block
@@ -548,17 +541,23 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
 #pragma unroll
                     for (int j = 0; j < V; ++j)
                         if ((cd[j] >> 6) == kTypeBoundary) {
-                            o[j] = boundary_update<C>(p.bs, e, n, pc[j] - qv[j], o[j]);
+                            if constexpr (S::kScaled) {
+                                const C dif = pc[j] * kphys - qv[j] * kphys;
+                                o[j] = boundary_update<C>(p.bs, e, n, dif, o[j] * kphys) / kphys;
+                            } else {
+                                o[j] = boundary_update<C>(p.bs, e, n, pc[j] - qv[j], o[j]);
+                            }
                             ++e;
                         }
                 }
             }
 
+            VT out;
 #pragma unroll
             for (int j = 0; j < V; ++j) {
                 if constexpr (S::kScaled) {
                     lmax = fmaxf(lmax, fabsf((float)o[j]));
-                    out.v[j] = store_round<T>(o[j] * sR, p.sr, (unsigned long long)(base + j), n, kSaltDense);
+                    out.v[j] = store_round<T>(o[j] * kout, p.sr, (unsigned long long)(base + j), n, kSaltDense);
                 } else {
                     out.v[j] = S::st(o[j]);
                 }
@@ -574,7 +573,7 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
 
     if constexpr (S::kScaled) {
         __shared__ float red[kWaves];
-        lmax = wave_max(lmax);
+        lmax = wave_max(lmax) * (float)kphys; // P units -> physical
         if (lane == 0) red[wave] = lmax;
         __syncthreads();
         if (threadIdx.x == 0) {

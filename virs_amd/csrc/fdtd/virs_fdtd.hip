@@ -278,17 +278,30 @@ int dense_grid(const virs_ctx *c, K kernel, int &zranges)
     return (int)std::max<long long>(1, std::min<long long>(g, W));
 }
 
+template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE>
+hipError_t launch_dense_k(const virs_ctx *c, const DenseParams &p)
+{
+    auto k = k_dense<T, V, ADJ, FUSED, SPONGE>;
+    DenseParams q = p;
+    const int grid = dense_grid(c, k, q.zranges);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, c->stream, q);
+    return hipGetLastError();
+}
+
 template <typename T, int V, bool ADJ>
 hipError_t launch_dense_v(const virs_ctx *c, const DenseParams &p)
 {
     if constexpr (sizeof(T) * V > 16) {
         return hipErrorInvalidValue;
     } else {
-        auto k = k_dense<T, V, ADJ>;
-        DenseParams q = p;
-        const int grid = dense_grid(c, k, q.zranges);
-        hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, c->stream, q);
-        return hipGetLastError();
+        const bool sponge = p.sp.w > 0;
+        if constexpr (!ADJ) {
+            if (p.bstart) {
+                return sponge ? launch_dense_k<T, V, false, true, true>(c, p)
+                              : launch_dense_k<T, V, false, true, false>(c, p);
+            }
+        }
+        return sponge ? launch_dense_k<T, V, ADJ, false, true>(c, p) : launch_dense_k<T, V, ADJ, false, false>(c, p);
     }
 }
 
@@ -475,7 +488,14 @@ int virs_create(const virs_desc *d, virs_ctx **out)
     }
     c->N = c->nx * c->ny * c->nz;
     c->esz = elem_size(d->precision);
-    const int maxv = d->precision == VIRS_FP64 ? 2 : d->precision == VIRS_FP32 ? 4 : 8;
+    int maxv = d->precision == VIRS_FP64 ? 2 : d->precision == VIRS_FP32 ? 4 : 8;
+    if (d->vector_width > 0) {
+        if ((d->vector_width & (d->vector_width - 1)) || d->vector_width > maxv || c->nx % d->vector_width) {
+            delete c;
+            return VIRS_ERR_ARG;
+        }
+        maxv = d->vector_width;
+    }
     c->vw = 1;
     for (int v = maxv; v > 1; v >>= 1)
         if (c->nx % v == 0) {
@@ -942,7 +962,7 @@ int virs_write_field_f64(virs_ctx *c, int32_t which, const double *in)
         if (mx > 0.0) {
             int k2;
             std::frexp(mx * 16.0, &k2);
-            e = std::min(std::max(14 - k2, -100), 100);
+            e = std::min(std::max(14 - k2, -60), 60);
         }
         StepCtl h;
         HIPC(c, hipMemcpy(&h, c->ctl, sizeof h, hipMemcpyDeviceToHost));

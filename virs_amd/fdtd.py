@@ -38,7 +38,7 @@ def dt_for_lambda2(lam2, h, c=343.0):
 class FDTD:
     def __init__(self, dims, h, dt=0.0, lam2=None, c=343.0, precision="fp32", sponge_width=0,
                  sponge_sigma=0.0, device=0, use_graph=False, dense_blocks=0, fuse_boundary=False,
-                 rounding="stochastic"):
+                 rounding="stochastic", vector_width=0):
         self.L = _lib.hip()
         if precision not in PRECISIONS:
             raise ValueError(f"precision must be one of {list(PRECISIONS)}")
@@ -60,6 +60,7 @@ class FDTD:
         if rounding not in ("nearest", "stochastic"):
             raise ValueError("rounding must be 'nearest' or 'stochastic'")
         d.rounding = 1 if rounding == "stochastic" else 0
+        d.vector_width = int(vector_width)
         self.desc = d
         self.dims = tuple(int(v) for v in dims)
         self.n = self.dims[0] * self.dims[1] * self.dims[2]
@@ -225,13 +226,13 @@ class FDTD:
 
 
 def setup(prep, precision="fp32", use_graph=False, dense_blocks=0, device=0, sponge=True,
-          fuse_boundary=False, rounding="stochastic"):
+          fuse_boundary=False, rounding="stochastic", vector_width=0):
     """A ready-to-run context for a rooms.Prepared workload."""
     sc = prep.scenario
     f = FDTD(prep.room.dims, sc.h, dt=prep.dt, precision=precision,
              sponge_width=sc.sponge_width if sponge else 0, sponge_sigma=sc.sponge_sigma,
              device=device, use_graph=use_graph, dense_blocks=dense_blocks,
-             fuse_boundary=fuse_boundary, rounding=rounding)
+             fuse_boundary=fuse_boundary, rounding=rounding, vector_width=vector_width)
     f.upload_room(prep.room)
     f.set_materials(sc.coeffs)
     f.set_soft_sources(prep.src_idx, prep.src_sig)
