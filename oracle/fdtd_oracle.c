@@ -389,7 +389,7 @@ int oracle_adjoint_f64(const oracle_desc *d, int64_t nsteps, const double *dL_di
  *     counter-based hash of (cell, step, salt) are added below the 16-bit
  *     format's last place before truncation (mix32 below);
  *   - the boundary list runs after the stencil on the stored field, soft
- *     sources after the boundary correction.
+ *     sources after the boundary correction on its stored result.
  * ------------------------------------------------------------------------ */
 
 /* 16 random bits for cell idx at step n: one 32-bit hash per cell pair,
@@ -397,8 +397,7 @@ int oracle_adjoint_f64(const oracle_desc *d, int64_t nsteps, const double *dL_di
 static uint32_t mix32(uint64_t idx, int64_t n, uint32_t salt)
 {
     const uint64_t pair = idx & ~(uint64_t)1;
-    uint32_t h = (uint32_t)pair * 0x9E3779B1u + (uint32_t)(pair >> 32) * 0x7FEB352Du +
-                 (uint32_t)n * 0x85EBCA77u + salt;
+    uint32_t h = (uint32_t)pair * 0x9E3779B1u + ((uint32_t)(pair >> 32) + (uint32_t)n) * 0x85EBCA77u + salt;
     h ^= h >> 16;
     h *= 0x2C1B3C6Du;
     h ^= h >> 15;
@@ -452,25 +451,37 @@ static uint16_t f32_to_bf16_rne(float f)
 
 static float bf16_to_f32(uint16_t b) { return u2f((uint32_t)b << 16); }
 
-/* Stochastic rounding: add r (16 random bits) to the magnitude below the
- * format's last place -- 13 bits for FP16, 16 for BF16 -- and truncate
- * them; the result is exactly representable except in FP16's subnormal
- * range, where the final conversion rounds to nearest even. */
+/* float -> binary16 rounding toward zero (finite overflow gives the largest
+ * finite half, as the hardware's packed round-toward-zero conversion does). */
+static uint16_t f32_to_f16_rtz(float f)
+{
+    const uint32_t u = f2u(f);
+    const uint32_t sign = (u >> 16) & 0x8000u;
+    const uint32_t mag = u & 0x7fffffffu;
+    if (mag > 0x7f800000u) return (uint16_t)(sign | 0x7e00u | ((mag >> 13) & 0x3ffu));
+    if (mag == 0x7f800000u) return (uint16_t)(sign | 0x7c00u);
+    if (mag >= 0x47800000u) return (uint16_t)(sign | 0x7bffu); /* >= 65536 */
+    if (mag >= 0x38800000u) return (uint16_t)(sign | (((mag >> 13) - (112u << 10)) & 0x7fffu));
+    if (mag < 0x33800000u) return (uint16_t)sign;              /* < 2^-24 */
+    const uint32_t e = mag >> 23;
+    const uint32_t m = (mag & 0x7fffffu) | 0x800000u;
+    return (uint16_t)(sign | (m >> (126u - e)));
+}
+
+/* Stochastic rounding: r (16 random bits, masked to the bits below the
+ * format's last place: 13 for FP16, 16 for BF16) is added to the float's bit
+ * pattern, the low bits are cleared and the result converted toward zero.
+ * In the normal range this is exactly the stochastic rounding of the value
+ * (the cleared pattern is representable); in FP16's subnormal range the
+ * conversion truncates. */
 static uint16_t f32_to_f16_sr(float f, uint32_t r)
 {
-    const uint32_t u = f2u(f), sign = u & 0x80000000u, mag = u & 0x7fffffffu;
-    if (mag >= 0x7f800000u) return f32_to_f16_rne(f);
-    const uint32_t mask = 0x1fffu;
-    const uint32_t mag2 = (mag + (r & mask)) & ~mask;
-    return f32_to_f16_rne(u2f(sign | mag2));
+    return f32_to_f16_rtz(u2f((f2u(f) + (r & 0x1fffu)) & 0xffffe000u));
 }
 
 static uint16_t f32_to_bf16_sr(float f, uint32_t r)
 {
-    const uint32_t u = f2u(f), sign = u & 0x80000000u, mag = u & 0x7fffffffu;
-    if (mag >= 0x7f800000u) return f32_to_bf16_rne(f);
-    const uint32_t mag2 = (mag + (r & 0xffffu)) & 0xffff0000u;
-    return (uint16_t)((sign | mag2) >> 16);
+    return (uint16_t)(((f2u(f) + (r & 0xffffu)) & 0xffff0000u) >> 16);
 }
 
 /* salts of the three places a cell may be stored within one step */
@@ -579,13 +590,12 @@ int oracle_run_s16(const oracle_desc *d, int bf16, int sr, int64_t nsteps, doubl
             s4[1] = s4[0]; s4[0] = in; s4[3] = s4[2]; s4[2] = yv;
             float r = ld16(&S, R[bi]) * sRi;
             r = (float)((double)r - (double)d->kconst * (yv * 0.1 + in * 0.9) * (double)0.4f);
-            for (int k = 0; k < d->nsoft; ++k)
-                if (bpos[k] == e && n < d->soft_len) r = r + (float)d->soft_sig[(int64_t)k * d->soft_len + n];
             if (fabsf(r) > mx) mx = fabsf(r);
             R[bi] = st16(&S, r * sR, (uint64_t)bi, n, SALT_BOUNDARY);
         }
+        /* soft sources, on boundary cells too, add to the stored value */
         for (int k = 0; k < d->nsoft; ++k) {
-            if (bpos[k] >= 0 || n >= d->soft_len) continue;
+            if (n >= d->soft_len) continue;
             const int64_t si = d->soft_idx[k];
             float r = ld16(&S, R[si]) * sRi;
             r = r + (float)d->soft_sig[(int64_t)k * d->soft_len + n];

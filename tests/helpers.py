@@ -37,6 +37,8 @@ def tiny_problem(seed=0, dims=(24, 18, 14), nmat=2, sponge=3, hard=True, nrcv=4,
     inside[3:nz - 3, 3:ny - 3, 3:nx - 3] = True
     air = np.flatnonzero((flags == 0) & inside)
     src = rng.choice(air, 2, replace=False)
+    # the second soft source sits on a boundary-list cell
+    src[1] = bidx[(2 * len(bidx)) // 3]
     rcv = list(rng.choice(air, max(nrcv - 2, 0), replace=False))
     rcv += [int(bidx[len(bidx) // 3]), int(src[0])]
     sig = np.stack([rooms.gauss_deriv(steps, 2.0, 10.0), rooms.unit_impulse(steps) * 0.5])

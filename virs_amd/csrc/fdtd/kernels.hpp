@@ -23,6 +23,7 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kSlots = 64;    // spread of the per-step |p| maximum atomics
 constexpr int kMaxSponge = 64;
 constexpr int kMaxPoints = 64;
+constexpr int kMaxFusedPlanes = 512; // planes per batch of a block's boundary phase
 
 // Device-resident step state.  nring[k] is the step index that the launches
 // of rotation phase k compute; each step's last launch writes the next
@@ -89,42 +90,70 @@ struct Store<bf16> {
 
 // Stochastic rounding to a 16-bit format.  A counter-based hash of
 // (cell pair, step, salt) gives 16 random bits per cell (one 32-bit hash
-// serves the two cells 2k and 2k+1); they are added to the float's
-// magnitude below the format's last place (13 bits for FP16, 16 for BF16)
-// and that part is then truncated, so E[stored] = value for every
-// normal-range result and increments smaller than half an ulp are not lost
-// step after step.  The truncated value is exactly representable except in
-// FP16's subnormal range, where the final conversion rounds to nearest.
-__device__ inline uint32_t mix32(unsigned long long idx, long long n, uint32_t salt)
+// serves the cells 2k and 2k+1: low half, high half).  They are masked to
+// the bits below the format's last place (13 for FP16, 16 for BF16), added
+// to the float's bit pattern, the low bits cleared, and the result converted
+// toward zero.  In the normal range that is exactly stochastic rounding:
+// E[stored] = value, so increments smaller than half an ulp are not lost
+// step after step; in FP16's subnormal range the conversion truncates.
+__device__ inline uint32_t hash_pair(uint32_t lo_k1, uint32_t hk)
 {
-    const unsigned long long pair = idx & ~1ull;
-    uint32_t h = (uint32_t)pair * 0x9E3779B1u + (uint32_t)(pair >> 32) * 0x7FEB352Du +
-                 (uint32_t)n * 0x85EBCA77u + salt;
+    uint32_t h = lo_k1 + hk;
     h ^= h >> 16;
     h *= 0x2C1B3C6Du;
     h ^= h >> 15;
+    return h;
+}
+constexpr uint32_t kHashK1 = 0x9E3779B1u, kHashK3 = 0x85EBCA77u;
+// hk term of the hash for cells idx (any of a pair) at step n
+__device__ inline uint32_t hash_hk(unsigned long long idx, long long n, uint32_t salt)
+{
+    return ((uint32_t)(idx >> 32) + (uint32_t)n) * kHashK3 + salt;
+}
+__device__ inline uint32_t mix16(unsigned long long idx, long long n, uint32_t salt)
+{
+    const unsigned long long pair = idx & ~1ull;
+    const uint32_t h = hash_pair((uint32_t)pair * kHashK1, hash_hk(pair, n, salt));
     return (idx & 1ull) ? (h >> 16) : (h & 0xffffu);
 }
 
 constexpr uint32_t kSaltDense = 0u, kSaltBoundary = 0x68E31DA4u, kSaltSource = 0xB5297A4Du;
 
+// Bit pattern of x stochastically rounded (random bits r16) to the 16-bit
+// format T, still as a float whose low bits are clear.
 template <typename T>
-__device__ inline T st_sr(float x, uint32_t r16)
+__device__ inline float sr_bits(float x, uint32_t r16)
 {
-    const uint32_t u = __float_as_uint(x), sign = u & 0x80000000u, mag = u & 0x7fffffffu;
-    if (mag >= 0x7f800000u) return Store<T>::st(x);
     constexpr uint32_t mask = __is_same(T, bf16) ? 0xffffu : 0x1fffu;
-    return Store<T>::st(__uint_as_float(sign | ((mag + (r16 & mask)) & ~mask)));
+    return __uint_as_float((__float_as_uint(x) + (r16 & mask)) & ~mask);
 }
 
-// Storage rounding of one value: nearest-even (FP32/FP64 always) or, for
-// the 16-bit formats when enabled, stochastic.
+// Converts two floats whose low bits are already clear to T.
+template <typename T>
+__device__ inline void cvt2_truncated(float a, float b, T &ta, T &tb)
+{
+    if constexpr (__is_same(T, bf16)) {
+        ta.bits = (uint16_t)(__float_as_uint(a) >> 16);
+        tb.bits = (uint16_t)(__float_as_uint(b) >> 16);
+    } else {
+        const auto h = __builtin_amdgcn_cvt_pkrtz(a, b);
+        ta = (T)h[0];
+        tb = (T)h[1];
+    }
+}
+
+// Single-value storage rounding (used off the hot loop): nearest-even, or
+// stochastic for the 16-bit formats when enabled.
 template <typename T>
 __device__ inline T store_round(typename Store<T>::C x, bool sr, unsigned long long idx, long long n,
                                 uint32_t salt)
 {
     if constexpr (Store<T>::kScaled) {
-        if (sr) return st_sr<T>(x, mix32(idx, n, salt));
+        if (sr) {
+            T a, b;
+            cvt2_truncated<T>(sr_bits<T>(x, mix16(idx, n, salt)), 0.f, a, b);
+            return a;
+        }
     }
     return Store<T>::st(x);
 }
@@ -240,6 +269,29 @@ __global__ void k_build_code(const uint8_t *__restrict__ flags, uint8_t *__restr
     }
 }
 
+// All-ones if bit `bit` of c is set, else zero (one signed bit-field extract).
+__device__ inline uint32_t bit_mask(uint32_t c, int bit) { return (uint32_t)__builtin_amdgcn_sbfe((int)c, bit, 1); }
+// Bitwise select, a where m is set, else b, as one v_bfi_b32 (the compiler
+// would otherwise rebuild a compare + conditional move from the mask).
+__device__ inline uint32_t bfi(uint32_t m, uint32_t a, uint32_t b)
+{
+    uint32_t r;
+    asm volatile("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(m), "v"(a), "v"(b));
+    return r;
+}
+__device__ inline float bsel(uint32_t m, float a, float b)
+{
+    return __uint_as_float(bfi(m, __float_as_uint(a), __float_as_uint(b)));
+}
+__device__ inline double bsel(uint32_t m, double a, double b)
+{
+    const unsigned long long ua = (unsigned long long)__double_as_longlong(a);
+    const unsigned long long ub = (unsigned long long)__double_as_longlong(b);
+    const unsigned long long lo = bfi(m, (uint32_t)ua, (uint32_t)ub);
+    const unsigned long long hi = bfi(m, (uint32_t)(ua >> 32), (uint32_t)(ub >> 32));
+    return __longlong_as_double((long long)(lo | (hi << 32)));
+}
+
 struct Sponge {
     int w;
     float a[kMaxSponge], b[kMaxSponge], g[kMaxSponge];
@@ -285,6 +337,7 @@ struct DenseParams {
     // fused boundary filter: entries of (chunk, z) are
     // [bstart[z*nchunks_plane + chunk], bstart[... + 1]) of the compact list
     const long long *bstart; // null: the boundary list runs in k_tail
+    const uint32_t *bidx;
     BoundaryState bs;
 };
 
@@ -326,8 +379,8 @@ __device__ inline C boundary_update(const BoundaryState &bs, long long e, long l
 // 16-bit storage: arithmetic runs in p^n's stored units (value * 2^eP);
 // power-of-two scaling commutes exactly with every rounding step, so the
 // results equal a computation in physical units bit for bit.
-// FUSED: the boundary biquad of the plane's list slice runs in the same
-// pass.  ADJ: the reverse sweep applies the same operator to the adjoint
+// FUSED: the boundary biquad of the block's list entries runs at the end of
+// the same launch.  ADJ: the reverse sweep applies the same operator to the adjoint
 // field; hard-source cells hold prescribed values, so their adjoint is zero.
 template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE>
 __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
@@ -389,7 +442,6 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
     const C c2dt2 = (C)p.c2dt2, ih2 = (C)p.ih2;
 
     __shared__ float sp_tab[SPONGE ? 3 : 1][SPONGE ? kMaxSponge : 1];
-    __shared__ int wsum[2][kWaves];
     if constexpr (SPONGE) {
         for (int k = threadIdx.x; k < p.sp.w; k += kBlock) {
             sp_tab[0][k] = p.sp.a[k];
@@ -398,7 +450,6 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
         }
         __syncthreads();
     }
-    int bparity = 0;
     float lmax = 0.f;
 
     // Work split over the (chunk, z) planes, flattened chunk-major.
@@ -419,9 +470,10 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
         wend = wbeg + per + ((long long)logical < rem ? 1 : 0);
     }
 
-    auto cvt = [&](const VT &t, C *out) {
-#pragma unroll
-        for (int j = 0; j < V; ++j) out[j] = S::ld(t.v[j]);
+    struct Stage {
+        VT pp, q, yn, ys; // P(z+1), Q(z), P(z,y+1), P(z,y-1) as stored
+        VB cd;            // cell codes of plane z
+        T tl, tr;         // P(z) left of lane 0 / right of lane 63
     };
 
     for (long long w = wbeg; w < wend;) {
@@ -435,67 +487,70 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
         const long long cell0 = active ? v * V : 0;
         const int y = (int)(cell0 / nx);
         const int x0 = (int)(cell0 - (long long)y * nx);
-        const bool has_yn = active && y + 1 < ny, has_ys = active && y > 0;
+        // out-of-grid neighbours read a valid cell instead; only border
+        // cells (written as zero) would use them
+        const long long dyn = (active && y + 1 < ny) ? nx : 0;
+        const long long dys = (active && y > 0) ? nx : 0;
         const bool need_l = active && lane == 0 && x0 > 0;
         const bool need_r = active && lane == 63 && x0 + V < nx;
 
-        // registers: pm/pc = P at z-1, z (P units); t* = raw loads of the
-        // plane computed next
-        C pm[V], pc[V];
-        VT tpp{}, tq{}, tyn{}, tys{};
-        VB tcd{};
-        T tl{}, tr{};
-#pragma unroll
-        for (int j = 0; j < V; ++j) pm[j] = pc[j] = (C)0;
-        if (active) {
-            if (z0 >= 1) cvt(vload<VT, false>(reinterpret_cast<const VT *>(P + (long long)(z0 - 1) * sxy + cell0)), pm);
-            cvt(vload<VT, false>(reinterpret_cast<const VT *>(P + (long long)z0 * sxy + cell0)), pc);
-        }
-        auto issue = [&](int z) {
+        auto issue = [&](int z, Stage &st) {
             const long long base = (long long)z * sxy + cell0;
-            if (active) {
-                if (z + 1 < nz) tpp = vload<VT, false>(reinterpret_cast<const VT *>(P + base + sxy));
-                tq = vload<VT, true>(reinterpret_cast<const VT *>(Q + base));
-                tcd = vload<VB, true>(reinterpret_cast<const VB *>(p.code + base));
-            }
-            if (has_yn) tyn = vload<VT, false>(reinterpret_cast<const VT *>(P + base + nx));
-            if (has_ys) tys = vload<VT, false>(reinterpret_cast<const VT *>(P + base - nx));
-            if (need_l) tl = P[base - 1];
-            if (need_r) tr = P[base + V];
+            st.pp = vload<VT, false>(reinterpret_cast<const VT *>(P + base + (z + 1 < nz ? sxy : 0)));
+            st.q = vload<VT, true>(reinterpret_cast<const VT *>(Q + base));
+            st.cd = vload<VB, true>(reinterpret_cast<const VB *>(p.code + base));
+            st.yn = vload<VT, false>(reinterpret_cast<const VT *>(P + base + dyn));
+            st.ys = vload<VT, false>(reinterpret_cast<const VT *>(P + base - dys));
+            if (need_l) st.tl = P[base - 1];
+            if (need_r) st.tr = P[base + V];
         };
-        issue(z0);
 
-        for (int z = z0; z < z1; ++z) {
-            const long long base = (long long)z * sxy + cell0;
-            C pp[V], qv[V], yn[V], ys[V];
-            uint8_t cd[V];
-            const bool has_zp = active && z + 1 < nz;
+        C B0[V], B1[V], B2[V];
+        Stage SA, SB;
+        {
+            const VT tm = vload<VT, false>(reinterpret_cast<const VT *>(P + (long long)max(z0 - 1, 0) * sxy + cell0));
+            const VT tc = vload<VT, false>(reinterpret_cast<const VT *>(P + (long long)z0 * sxy + cell0));
 #pragma unroll
             for (int j = 0; j < V; ++j) {
-                pp[j] = has_zp ? S::ld(tpp.v[j]) : (C)0;
-                if constexpr (S::kScaled) qv[j] = S::ld(tq.v[j]) * kq;
-                else qv[j] = S::ld(tq.v[j]);
-                yn[j] = S::ld(tyn.v[j]);
-                ys[j] = S::ld(tys.v[j]);
-                cd[j] = active ? tcd.v[j] : (uint8_t)(kTypeZero << 6);
+                B0[j] = z0 >= 1 ? S::ld(tm.v[j]) : (C)0;
+                B1[j] = S::ld(tc.v[j]);
             }
-            C left = __shfl_up(pc[V - 1], 1);
-            C right = __shfl_down(pc[0], 1);
-            if (need_l) left = S::ld(tl);
-            if (need_r) right = S::ld(tr);
-            if (z + 1 < z1) issue(z + 1);
+        }
+        issue(z0, SA);
+
+        // One plane: PM/PC hold P(z-1)/P(z); PP receives P(z+1) from cur;
+        // nxt receives the loads of plane z+1 before plane z is computed.
+        auto body = [&](const int z, C(&PM)[V], C(&PC)[V], C(&PP)[V], Stage &cur, Stage &nxt) {
+            const long long base = (long long)z * sxy + cell0;
+            const bool zp_ok = z + 1 < nz;
+            C qv[V], yn[V], ys[V];
+            uint32_t cd[V];
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                PP[j] = zp_ok ? S::ld(cur.pp.v[j]) : (C)0;
+                if constexpr (S::kScaled) qv[j] = S::ld(cur.q.v[j]) * kq;
+                else qv[j] = S::ld(cur.q.v[j]);
+                yn[j] = S::ld(cur.yn.v[j]);
+                ys[j] = S::ld(cur.ys.v[j]);
+                cd[j] = active ? (uint32_t)cur.cd.v[j] : (uint32_t)(kTypeZero << 6);
+            }
+            C left = __shfl_up(PC[V - 1], 1);
+            C right = __shfl_down(PC[0], 1);
+            if (need_l) left = S::ld(cur.tl);
+            if (need_r) right = S::ld(cur.tr);
+            if (z + 1 < z1) issue(z + 1, nxt);
 
             C o[V];
 #pragma unroll
             for (int j = 0; j < V; ++j) {
                 const uint32_t c = cd[j];
-                const C pcv = pc[j];
-                const C xp = (c & 1u) ? pcv : (j + 1 < V ? pc[(j + 1) % V] : right);
-                const C xm = (c & 2u) ? pcv : (j > 0 ? pc[(j + V - 1) % V] : left);
-                const C yp = (c & 4u) ? pcv : yn[j];
-                const C ym = (c & 8u) ? pcv : ys[j];
-                const C zp = (c & 16u) ? pcv : pp[j];
-                const C zm = (c & 32u) ? pcv : pm[j];
+                const C pcv = PC[j];
+                const C xp = bsel(bit_mask(c, 0), pcv, j + 1 < V ? PC[(j + 1) % V] : right);
+                const C xm = bsel(bit_mask(c, 1), pcv, j > 0 ? PC[(j + V - 1) % V] : left);
+                const C yp = bsel(bit_mask(c, 2), pcv, yn[j]);
+                const C ym = bsel(bit_mask(c, 3), pcv, ys[j]);
+                const C zp = bsel(bit_mask(c, 4), pcv, PP[j]);
+                const C zm = bsel(bit_mask(c, 5), pcv, PM[j]);
                 C sum = xp + xm;
                 sum = sum + yp;
                 sum = sum + ym;
@@ -512,62 +567,112 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
                     }
                 }
                 const C upd = (a * pcv - b * qv[j]) + g * (c2dt2 * lap);
-                const uint32_t type = c >> 6;
-                // update (0) or update + boundary filter (3); hard source (2); zero (1)
-                o[j] = ((type + 1u) & 2u) == 0u ? upd : (type == kTypeHard ? hard_u : (C)0);
-            }
-
-            if constexpr (FUSED) {
-                const long long t = (long long)z * p.nchunks_plane + pchunk;
-                const long long eb = p.bstart[t], ee = p.bstart[t + 1];
-                if (eb < ee) { // uniform over the block
-                    int cnt = 0;
-#pragma unroll
-                    for (int j = 0; j < V; ++j) cnt += (cd[j] >> 6) == kTypeBoundary;
-                    // exclusive prefix of cnt over the block, lane order = cell order
-                    const unsigned long long lt = (1ull << lane) - 1ull;
-                    int pre = 0, tot = 0;
-#pragma unroll
-                    for (int bit = 0; bit < 4; ++bit) {
-                        const unsigned long long m = __ballot((cnt >> bit) & 1);
-                        pre += __popcll(m & lt) << bit;
-                        tot += __popcll(m) << bit;
-                    }
-                    if (lane == 0) wsum[bparity][wave] = tot;
-                    __syncthreads();
-                    for (int k = 0; k < wave; ++k) pre += wsum[bparity][k];
-                    bparity ^= 1;
-                    long long e = eb + pre;
-#pragma unroll
-                    for (int j = 0; j < V; ++j)
-                        if ((cd[j] >> 6) == kTypeBoundary) {
-                            if constexpr (S::kScaled) {
-                                const C dif = pc[j] * kphys - qv[j] * kphys;
-                                o[j] = boundary_update<C>(p.bs, e, n, dif, o[j] * kphys) / kphys;
-                            } else {
-                                o[j] = boundary_update<C>(p.bs, e, n, pc[j] - qv[j], o[j]);
-                            }
-                            ++e;
-                        }
-                }
+                // type bits 6-7: 00 update, 11 update + filter, 10 hard source, 01 zero
+                const uint32_t differ = bit_mask(c ^ (c >> 1), 6);
+                const uint32_t is_hard = differ & bit_mask(c, 7);
+                o[j] = bsel(differ, bsel(is_hard, hard_u, (C)0), upd);
             }
 
             VT out;
+            if constexpr (S::kScaled) {
 #pragma unroll
-            for (int j = 0; j < V; ++j) {
-                if constexpr (S::kScaled) {
-                    lmax = fmaxf(lmax, fabsf((float)o[j]));
-                    out.v[j] = store_round<T>(o[j] * kout, p.sr, (unsigned long long)(base + j), n, kSaltDense);
+                for (int j = 0; j < V; ++j) lmax = fmaxf(lmax, fabsf((float)o[j]));
+                if (p.sr) {
+                    const uint32_t hk = hash_hk((unsigned long long)(base & ~1ll), n, kSaltDense);
+                    const uint32_t lo = (uint32_t)(base & ~1ll) * kHashK1;
+                    if constexpr (V == 1) {
+                        const uint32_t h = hash_pair(lo, hk);
+                        const float q0 = sr_bits<T>(o[0] * kout, (base & 1) ? (h >> 16) : (h & 0xffffu));
+                        T dummy;
+                        cvt2_truncated<T>(q0, 0.f, out.v[0], dummy);
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < V; j += 2) {
+                            const uint32_t h = hash_pair(lo + (uint32_t)j * kHashK1, hk);
+                            const float qa = sr_bits<T>(o[j] * kout, h & 0xffffu);
+                            const float qb = sr_bits<T>(o[j + 1] * kout, h >> 16);
+                            cvt2_truncated<T>(qa, qb, out.v[j], out.v[j + 1]);
+                        }
+                    }
                 } else {
-                    out.v[j] = S::st(o[j]);
+#pragma unroll
+                    for (int j = 0; j < V; ++j) out.v[j] = S::st(o[j] * kout);
                 }
+            } else {
+#pragma unroll
+                for (int j = 0; j < V; ++j) out.v[j] = S::st(o[j]);
             }
             if (active) vstore<VT, true>(reinterpret_cast<VT *>(R + base), out);
-#pragma unroll
-            for (int j = 0; j < V; ++j) {
-                pm[j] = pc[j];
-                pc[j] = pp[j];
+        };
+
+        for (int z = z0; z < z1; z += 6) {
+            body(z, B0, B1, B2, SA, SB);
+            if (z + 1 >= z1) break;
+            body(z + 1, B1, B2, B0, SB, SA);
+            if (z + 2 >= z1) break;
+            body(z + 2, B2, B0, B1, SA, SB);
+            if (z + 3 >= z1) break;
+            body(z + 3, B0, B1, B2, SB, SA);
+            if (z + 4 >= z1) break;
+            body(z + 4, B1, B2, B0, SA, SB);
+            if (z + 5 >= z1) break;
+            body(z + 5, B2, B0, B1, SB, SA);
+        }
+    }
+
+    // FUSED: the boundary biquad for the list entries inside this block's
+    // (chunk, z) planes.  Each such cell was written above by this block and
+    // its correction reads only p^n, p^{n-1} and that value, so a block-level
+    // barrier orders it -- no cross-block dependency.  All the block's
+    // entries are handled as one batch (virtual index over its planes'
+    // list slices), and each is stored exactly as the separate list launch
+    // would store it.
+    if constexpr (FUSED) {
+        __shared__ long long pstart[kMaxFusedPlanes + 1];
+        __shared__ long long pbase[kMaxFusedPlanes];
+        const long long np = wend - wbeg;
+        __syncthreads();
+        for (long long k0 = 0; k0 < np; k0 += kMaxFusedPlanes) {
+            const int cnt = (int)min((long long)kMaxFusedPlanes, np - k0);
+            if (threadIdx.x == 0) {
+                long long acc = 0;
+                for (int k = 0; k < cnt; ++k) {
+                    const long long w = wbeg + k0 + k;
+                    const long long ch = w / nz, zz = w - ch * nz;
+                    const long long t = zz * p.nchunks_plane + ch;
+                    pstart[k] = acc;
+                    pbase[k] = p.bstart[t];
+                    acc += p.bstart[t + 1] - p.bstart[t];
+                }
+                pstart[cnt] = acc;
             }
+            __syncthreads();
+            const long long total = pstart[cnt];
+            for (long long vi = threadIdx.x; vi < total; vi += kBlock) {
+                int lo = 0, hi = cnt; // largest k with pstart[k] <= vi
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (pstart[mid] <= vi) lo = mid;
+                    else hi = mid;
+                }
+                const long long e = pbase[lo] + (vi - pstart[lo]);
+                const long long b = p.bidx[e];
+                C pv = S::ld(P[b]), qv = S::ld(Q[b]), r = S::ld(R[b]);
+                if constexpr (S::kScaled) {
+                    // physical units for the filter and the correction
+                    pv = pv * kphys;
+                    qv = qv * kphys * kq;
+                    r = r * kphys / kout;
+                }
+                r = boundary_update<C>(p.bs, e, n, pv - qv, r);
+                if constexpr (S::kScaled) {
+                    lmax = fmaxf(lmax, fabsf((float)(r / kphys)));
+                    R[b] = store_round<T>(r * (kout / kphys), p.sr, (unsigned long long)b, n, kSaltBoundary);
+                } else {
+                    R[b] = S::st(r);
+                }
+            }
+            __syncthreads();
         }
     }
 
@@ -653,17 +758,20 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
         C pv = S::ld(P[b]), qv = S::ld(Q[b]), r = S::ld(R[b]);
         if constexpr (S::kScaled) pv = pv * sP, qv = qv * sQ, r = r * sRi;
         r = boundary_update<C>(p.bs, e, n, pv - qv, r);
+        T t = store_round<T>(r * sR, p.sr, (unsigned long long)b, n, kSaltBoundary);
+        if constexpr (S::kScaled) lmax = fabsf((float)r);
+        // a soft source on this cell is added to the stored corrected value
         for (int k = 0; k < p.nsoft; ++k)
-            if (p.soft_bpos[k] == e && in_sig) r = r + (C)p.soft_sig[(long long)k * p.soft_len + n];
-        C stored = r;
-        if constexpr (S::kScaled) {
-            const T t = store_round<T>(r * sR, p.sr, (unsigned long long)b, n, kSaltBoundary);
-            R[b] = t;
-            stored = S::ld(t) * sRi;
-            lmax = fabsf((float)r);
-        } else {
-            R[b] = S::st(r);
-        }
+            if (p.soft_bpos[k] == e && in_sig) {
+                C r2 = S::ld(t);
+                if constexpr (S::kScaled) r2 = r2 * sRi;
+                r2 = r2 + (C)p.soft_sig[(long long)k * p.soft_len + n];
+                if constexpr (S::kScaled) lmax = fmaxf(lmax, fabsf((float)r2));
+                t = store_round<T>(r2 * sR, p.sr, (unsigned long long)b, n, kSaltSource);
+            }
+        R[b] = t;
+        C stored = S::ld(t);
+        if constexpr (S::kScaled) stored = stored * sRi;
         if (in_ir)
             for (int k = 0; k < p.nrcv; ++k)
                 if (p.rcv_bpos[k] == e) p.ir[(long long)k * p.ir_cap + n] = (double)stored;
@@ -675,12 +783,8 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
             C r = S::ld(R[i]);
             if constexpr (S::kScaled) r = r * sRi;
             r = r + (C)p.soft_sig[(long long)k * p.soft_len + n];
-            if constexpr (S::kScaled) {
-                R[i] = store_round<T>(r * sR, p.sr, (unsigned long long)i, n, kSaltSource);
-                lmax = fmaxf(lmax, fabsf((float)r));
-            } else {
-                R[i] = S::st(r);
-            }
+            if constexpr (S::kScaled) lmax = fmaxf(lmax, fabsf((float)r));
+            R[i] = store_round<T>(r * sR, p.sr, (unsigned long long)i, n, kSaltSource);
         }
         __syncthreads();
         if (k < p.nrcv && p.rcv_bpos[k] < 0 && in_ir) {

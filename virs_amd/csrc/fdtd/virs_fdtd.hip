@@ -362,6 +362,7 @@ int launch_step(virs_ctx *c, double *trace_d = nullptr, double *trace_y = nullpt
     bs.trace_slots = c->trace_slots;
     if (c->fused) {
         p.bstart = c->bstart;
+        p.bidx = c->bidx;
         t.bs.nb = 0;
         t.soft_bpos = c->neg;
         t.rcv_bpos = c->neg;
@@ -379,15 +380,13 @@ int launch_step(virs_ctx *c, double *trace_d = nullptr, double *trace_y = nullpt
     return VIRS_OK;
 }
 
-// Decides whether the boundary filter can run inside the stencil (the list
-// must be exactly the interior flag-2 cells, in order) and builds the
-// per-(chunk, z) offsets into the list.
+// Builds the per-(chunk, z) offsets into the ascending boundary list that
+// let the stencil launch run the boundary filter of its own planes.
 int refresh_fusion(virs_ctx *c)
 {
     dfree(c->bstart);
     c->fused = false;
-    if (!c->d.fuse_boundary || !c->have_flags || c->N > (long long)UINT32_MAX || c->h_bidx != c->h_flag2)
-        return VIRS_OK;
+    if (!c->d.fuse_boundary || c->nb == 0) return VIRS_OK;
     const long long nch = c->nchunks_plane, chunk_cells = (long long)kBlock * c->vw;
     const long long sxy = c->nx * c->ny;
     std::vector<long long> cnt((size_t)(c->nz * nch + 1), 0);
