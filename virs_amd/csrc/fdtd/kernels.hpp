@@ -23,7 +23,6 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kSlots = 64;    // spread of the per-step |p| maximum atomics
 constexpr int kMaxSponge = 64;
 constexpr int kMaxPoints = 64;
-constexpr int kMaxFusedPlanes = 512; // planes per batch of a block's boundary phase
 
 // Device-resident step state.  nring[k] is the step index that the launches
 // of rotation phase k compute; each step's last launch writes the next
@@ -344,13 +343,14 @@ struct DenseParams {
 // DF1 biquad step and boundary correction for list entry e (simulator_gpu.cu:126-148):
 //   y = b0 d + b1 x1 + b2 x2 - a1 y1 - a2 y2   (FP64 state)
 //   R = R - K (0.1 y + 0.9 d) 0.4f             (evaluated in double)
+// x1..y2 is the entry's state as loaded before the call.
 template <typename C>
-__device__ inline C boundary_update(const BoundaryState &bs, long long e, long long n, C dif, C r)
+__device__ inline C boundary_update_pre(const BoundaryState &bs, long long e, long long n, C dif, C r,
+                                        double x1, double x2, double y1, double y2)
 {
     const int m = bs.bmat ? bs.bmat[e] : 0;
     const double *cf = bs.coeffs + 5 * m;
     const double in = (double)dif;
-    const double x1 = bs.s0[e], x2 = bs.s1[e], y1 = bs.s2[e], y2 = bs.s3[e];
     const double yv = cf[0] * in + cf[1] * x1 + cf[2] * x2 - cf[3] * y1 - cf[4] * y2;
     bs.s1[e] = x1;
     bs.s0[e] = in;
@@ -364,6 +364,12 @@ __device__ inline C boundary_update(const BoundaryState &bs, long long e, long l
         }
     }
     return (C)((double)r - (double)bs.kconst * (yv * 0.1 + in * 0.9) * (double)0.4f);
+}
+
+template <typename C>
+__device__ inline C boundary_update(const BoundaryState &bs, long long e, long long n, C dif, C r)
+{
+    return boundary_update_pre<C>(bs, e, n, dif, r, bs.s0[e], bs.s1[e], bs.s2[e], bs.s3[e]);
 }
 
 // Leapfrog 7-point update, simulator_gpu.cu:38-103 with a graded sponge:
@@ -628,48 +634,88 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
     // list slices), and each is stored exactly as the separate list launch
     // would store it.
     if constexpr (FUSED) {
-        __shared__ long long pstart[kMaxFusedPlanes + 1];
-        __shared__ long long pbase[kMaxFusedPlanes];
+        __shared__ long long pstart[kBlock + 1];
+        __shared__ long long pbase[kBlock];
+        __shared__ long long wtot[kWaves];
         const long long np = wend - wbeg;
         __syncthreads();
-        for (long long k0 = 0; k0 < np; k0 += kMaxFusedPlanes) {
-            const int cnt = (int)min((long long)kMaxFusedPlanes, np - k0);
-            if (threadIdx.x == 0) {
-                long long acc = 0;
-                for (int k = 0; k < cnt; ++k) {
-                    const long long w = wbeg + k0 + k;
-                    const long long ch = w / nz, zz = w - ch * nz;
-                    const long long t = zz * p.nchunks_plane + ch;
-                    pstart[k] = acc;
-                    pbase[k] = p.bstart[t];
-                    acc += p.bstart[t + 1] - p.bstart[t];
-                }
-                pstart[cnt] = acc;
+        for (long long k0 = 0; k0 < np; k0 += kBlock) {
+            const int cnt = (int)min((long long)kBlock, np - k0);
+            // thread k: list slice of plane k of the batch; block-wide
+            // exclusive scan of the slice lengths
+            long long len = 0, base0 = 0;
+            if ((int)threadIdx.x < cnt) {
+                const long long w = wbeg + k0 + threadIdx.x;
+                const long long ch = w / nz, zz = w - ch * nz;
+                const long long t = zz * p.nchunks_plane + ch;
+                base0 = p.bstart[t];
+                len = p.bstart[t + 1] - base0;
             }
+            long long inc = len;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const long long up = __shfl_up(inc, o);
+                if (lane >= o) inc += up;
+            }
+            if (lane == 63) wtot[wave] = inc;
             __syncthreads();
-            const long long total = pstart[cnt];
-            for (long long vi = threadIdx.x; vi < total; vi += kBlock) {
-                int lo = 0, hi = cnt; // largest k with pstart[k] <= vi
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (pstart[mid] <= vi) lo = mid;
-                    else hi = mid;
+            long long off = 0;
+            for (int k = 0; k < wave; ++k) off += wtot[k];
+            pstart[threadIdx.x] = off + inc - len;
+            pbase[threadIdx.x] = base0;
+            if (threadIdx.x == kBlock - 1) pstart[kBlock] = off + inc;
+            __syncthreads();
+            const long long total = pstart[kBlock];
+            // entries in batches of U per thread: all loads of a batch are
+            // issued before any of its stores
+            constexpr int U = 4;
+            for (long long v0 = threadIdx.x; v0 < total; v0 += (long long)kBlock * U) {
+                long long ev[U], bv[U];
+                bool ok[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const long long vi = v0 + (long long)u * kBlock;
+                    ok[u] = vi < total;
+                    int lo = 0, hi = cnt; // largest k with pstart[k] <= vi
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (pstart[mid] <= vi) lo = mid;
+                        else hi = mid;
+                    }
+                    ev[u] = ok[u] ? pbase[lo] + (vi - pstart[lo]) : 0;
                 }
-                const long long e = pbase[lo] + (vi - pstart[lo]);
-                const long long b = p.bidx[e];
-                C pv = S::ld(P[b]), qv = S::ld(Q[b]), r = S::ld(R[b]);
-                if constexpr (S::kScaled) {
-                    // physical units for the filter and the correction
-                    pv = pv * kphys;
-                    qv = qv * kphys * kq;
-                    r = r * kphys / kout;
+#pragma unroll
+                for (int u = 0; u < U; ++u) bv[u] = ok[u] ? (long long)p.bidx[ev[u]] : 0;
+                C pv[U], qv2[U], rv[U];
+                double x1[U], x2[U], y1[U], y2[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    pv[u] = S::ld(P[bv[u]]);
+                    qv2[u] = S::ld(Q[bv[u]]);
+                    rv[u] = S::ld(R[bv[u]]);
+                    x1[u] = p.bs.s0[ev[u]];
+                    x2[u] = p.bs.s1[ev[u]];
+                    y1[u] = p.bs.s2[ev[u]];
+                    y2[u] = p.bs.s3[ev[u]];
                 }
-                r = boundary_update<C>(p.bs, e, n, pv - qv, r);
-                if constexpr (S::kScaled) {
-                    lmax = fmaxf(lmax, fabsf((float)(r / kphys)));
-                    R[b] = store_round<T>(r * (kout / kphys), p.sr, (unsigned long long)b, n, kSaltBoundary);
-                } else {
-                    R[b] = S::st(r);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (!ok[u]) continue;
+                    C pp = pv[u], qq = qv2[u], r = rv[u];
+                    if constexpr (S::kScaled) {
+                        // physical units for the filter and the correction
+                        pp = pp * kphys;
+                        qq = qq * kphys * kq;
+                        r = r * kphys / kout;
+                    }
+                    r = boundary_update_pre<C>(p.bs, ev[u], n, pp - qq, r, x1[u], x2[u], y1[u], y2[u]);
+                    if constexpr (S::kScaled) {
+                        lmax = fmaxf(lmax, fabsf((float)(r / kphys)));
+                        R[bv[u]] = store_round<T>(r * (kout / kphys), p.sr, (unsigned long long)bv[u], n,
+                                                  kSaltBoundary);
+                    } else {
+                        R[bv[u]] = S::st(r);
+                    }
                 }
             }
             __syncthreads();
