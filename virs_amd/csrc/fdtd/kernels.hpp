@@ -668,7 +668,7 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
             const long long total = pstart[kBlock];
             // entries in batches of U per thread: all loads of a batch are
             // issued before any of its stores
-            constexpr int U = 4;
+            constexpr int U = 1; // more entries per batch raise the whole kernel's VGPR count
             for (long long v0 = threadIdx.x; v0 < total; v0 += (long long)kBlock * U) {
                 long long ev[U], bv[U];
                 bool ok[U];
