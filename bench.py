@@ -28,6 +28,22 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Gvoxel-updates/s + % HBM peak; FP16-store vs FP32 at ≤1e-3 IR err vs FP64"
 
 
+PROFILE = os.path.join(ROOT, "profiles", "k_dense_random_512x384x256.md")
+
+
+def measured_traffic(precision):
+    """HBM bytes per stencil launch on this workload from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE runs (calibrated against a plain copy),
+    or None when the profile does not cover this precision."""
+    try:
+        for line in open(PROFILE):
+            if line.startswith(f"traffic_bytes_per_launch {precision} k_dense:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def host_cores():
     try:
         return len(os.sched_getaffinity(0))
@@ -149,7 +165,9 @@ def main():
                                "8 receivers, Gaussian-derivative source)",
                    "cells": n, "boundary_cells": nb, "storage": args.precision},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(args.precision) if not args.fuse and not args.dense_blocks else None,
+                     "traffic_source": "profiles/k_dense_random_512x384x256.md (rocprofv3 --pmc, copy-calibrated)",
                      "kernel": "k_dense", "kernel_ms": round(r["dense_ms"], 5),
                      "bytes_per_launch": dense_bytes},
         "step_algorithmic_gbs": round(sbytes / (ms / 1e3) / 1e9, 1),
