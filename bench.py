@@ -58,7 +58,10 @@ def step_bytes(n, nb, nrcv, s):
 
 def run_gpu(prep, precision, steps, warmup, dense_blocks=0, fuse=False, vw=0):
     from virs_amd.fdtd import BYTES, setup
+    ts = time.perf_counter()
     f = setup(prep, precision, dense_blocks=dense_blocks, fuse_boundary=fuse, vector_width=vw)
+    f.synchronize()
+    setup_s = time.perf_counter() - ts
     f.start(warmup + steps)
     if warmup:
         f.step(warmup)
@@ -67,11 +70,13 @@ def run_gpu(prep, precision, steps, warmup, dense_blocks=0, fuse=False, vw=0):
     total_ms, dense_ms, bnd_ms = f.time_steps(steps)
     f.synchronize()
     wall = time.perf_counter() - t0
-    ir = f.receivers(True)
+    tr = time.perf_counter()
+    ir = f.receivers()
+    read_s = time.perf_counter() - tr
     mem = f.memory_bytes()
     f.close()
     return dict(total_ms=total_ms, dense_ms=dense_ms, bnd_ms=bnd_ms, wall_s=wall, ir=ir,
-                mem=mem, s=BYTES[precision])
+                mem=mem, s=BYTES[precision], setup_s=setup_s, read_s=read_s)
 
 
 def cpu_baseline(prep, crop=128, steps=200, warmup=5):
@@ -174,6 +179,10 @@ def main():
         "step_hbm_frac": round(sbytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
         "boundary_kernel_ms": round(r["bnd_ms"], 5),
         "device_bytes": r["mem"],
+        # context creation + host->device upload of flags, normals, list and
+        # tables, and the receiver readback, added to the timed loop
+        "pcie_inclusive": {"setup_s": round(r["setup_s"], 4), "readback_s": round(r["read_s"], 5),
+                           "value": round(n * args.steps / (r["total_ms"] / 1e3 + r["setup_s"] + r["read_s"]) / 1e9, 3)},
         "ir_finite": bool(np.isfinite(r["ir"]).all()),
     }
     variants = {}

@@ -128,12 +128,14 @@ float virs_percent_to_target_rms(const virs_ctx *ctx);
 int virs_read_receivers(virs_ctx *ctx, float *out);
 int virs_read_receivers_f64(virs_ctx *ctx, double *out);
 /* updateCurrentGridFromGPU / updateAllGridFromGPU (simulator_dispatch.cpp:31-49):
- * which = 0 current field p^n, 1 previous p^{n-1}; values in physical units. */
+ * which = 0 current field p^n, 1 previous p^{n-1}, 2 the third buffer (p^{n-2},
+ * the reference's p_next after its pointer rotation); values in physical units. */
 int virs_read_field(virs_ctx *ctx, int32_t which, float *out);
 int virs_read_field_f64(virs_ctx *ctx, int32_t which, double *out);
 /* updateGPUFromGrid, field part: set p^n (which 0) or p^{n-1} (which 1). */
 int virs_write_field_f64(virs_ctx *ctx, int32_t which, const double *in);
-/* Power-of-two exponent e of a stored field (stored = value * 2^e); 0 unless FP16/BF16. */
+/* Power-of-two exponent e of a stored field (which as for virs_read_field;
+ * stored = value * 2^e); 0 unless FP16/BF16. */
 int virs_field_exponent(virs_ctx *ctx, int32_t which, int32_t *e);
 
 /* Device bytes held by the context. */

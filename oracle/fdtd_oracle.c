@@ -505,6 +505,22 @@ static uint16_t st16(const store16 *s, float x, uint64_t idx, int64_t n, uint32_
     return s->bf16 ? f32_to_bf16_rne(x) : f32_to_f16_rne(x);
 }
 
+/* The storage rounding of the 16-bit runner on n floats: rbits (16 random
+ * bits per value) selects stochastic rounding, NULL round-to-nearest-even.
+ * out receives the 16-bit patterns, back (may be NULL) their float values. */
+void oracle_round16(const float *in, const uint32_t *rbits, int64_t n, int bf16, uint16_t *out,
+                    float *back)
+{
+    const store16 S = {bf16, rbits != NULL};
+    for (int64_t i = 0; i < n; ++i) {
+        uint16_t v;
+        if (rbits) v = bf16 ? f32_to_bf16_sr(in[i], rbits[i]) : f32_to_f16_sr(in[i], rbits[i]);
+        else v = bf16 ? f32_to_bf16_rne(in[i]) : f32_to_f16_rne(in[i]);
+        out[i] = v;
+        if (back) back[i] = ld16(&S, v);
+    }
+}
+
 static float exp2i_f(int e) { return u2f((uint32_t)(e + 127) << 23); }
 
 int oracle_run_s16(const oracle_desc *d, int bf16, int sr, int64_t nsteps, double *ir,

@@ -61,6 +61,8 @@ def lib():
         L.oracle_run_s16.restype = C.c_int
         L.oracle_run_s16.argtypes = [C.POINTER(OracleDesc), C.c_int, C.c_int, C.c_int64, dp, dp, dp,
                                      C.POINTER(C.c_int32)]
+        L.oracle_round16.restype = None
+        L.oracle_round16.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p]
         L.oracle_set_threads.restype = None
         L.oracle_set_threads.argtypes = [C.c_int]
         _lib = L
@@ -212,6 +214,21 @@ class Problem:
         if rc != 0:
             raise RuntimeError(f"oracle adjoint failed: {rc}")
         return g
+
+
+def round16(x, fmt="fp16", rbits=None):
+    """Storage rounding of the 16-bit runner: (uint16 patterns, float32 values).
+    rbits (16 random bits per value) selects stochastic rounding."""
+    L = lib()
+    x = np.ascontiguousarray(np.asarray(x, np.float32).reshape(-1))
+    out = np.zeros(x.size, np.uint16)
+    back = np.zeros(x.size, np.float32)
+    rb = None
+    if rbits is not None:
+        rb = np.ascontiguousarray(np.broadcast_to(np.asarray(rbits, np.uint32), x.shape))
+    L.oracle_round16(x.ctypes.data, None if rb is None else rb.ctypes.data, x.size,
+                     int(fmt == "bf16"), out.ctypes.data, back.ctypes.data)
+    return out, back
 
 
 def rms_continue_sequence(ir, target=0.0009):

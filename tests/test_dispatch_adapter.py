@@ -1,0 +1,158 @@
+"""The dispatch adapter (include/virs_dispatch.hpp): the reference's 16
+dispatch functions (simulator_dispatch.h:4-25) on the C ABI, driven in
+Simulator's call order through a stand-in VectorSpace
+(tests/adapter/dispatch_driver.cpp), checked against the oracle."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from helpers import tiny_problem
+from oracle import oracle as O
+from virs_amd import grid as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "adapter", "_build", "libdispatch_driver.so")
+
+# simulator_dispatch.h:4-25
+REFERENCE_DISPATCH = [
+    "fdtd_setup", "fdtd_step", "fdtd_cleanup", "fdtd_start_simulation", "initPressureSphere",
+    "buildSpongeLayer", "updateAllGridFromGPU", "updateCurrentGridFromGPU", "updateGPUFromGrid",
+    "uploadNormalsToGPU", "uploadPZetaToGPU", "uploadBoundaryIndicesToGPU", "allocFilterStates",
+    "allocFilterCoeffs", "uploadFilterCoeffsToGPU",
+]
+
+
+def driver():
+    if not os.path.exists(DRIVER):
+        raise RuntimeError(f"{DRIVER} is missing: run __graft_entry__.build() first")
+    L = C.CDLL(DRIVER)
+    f = L.virs_drive_reference_sequence
+    f.restype = C.c_int
+    f.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_float, C.c_void_p, C.c_int64,
+                  C.c_void_p, C.c_float, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                  C.c_void_p, C.c_void_p, C.c_void_p]
+    return f
+
+
+def run_sequence(flags, h, coeffs5, steps, receiver, sphere=None, pressure=0.0):
+    nz, ny, nx = flags.shape
+    os.environ["VIRS_RECEIVER"] = ",".join(str(int(v)) for v in receiver)
+    flags = np.ascontiguousarray(flags, np.uint8)
+    co = np.ascontiguousarray(coeffs5, np.float32)
+    out = dict(flags=np.zeros_like(flags), ir=np.zeros(max(steps, 1), np.float32),
+               p=np.zeros(flags.shape, np.float32), q=np.zeros(flags.shape, np.float32),
+               r=np.zeros(flags.shape, np.float32))
+    sp = None if sphere is None else np.ascontiguousarray(sphere, np.int64)
+    run = C.c_int64(0)
+    pct = C.c_float(0)
+    nb = C.c_int64(0)
+    rc = driver()(flags.ctypes.data, nx, ny, nz, h, co.ctypes.data, steps,
+                  None if sp is None else sp.ctypes.data, pressure, out["flags"].ctypes.data,
+                  out["ir"].ctypes.data, out["p"].ctypes.data, out["q"].ctypes.data,
+                  out["r"].ctypes.data, C.byref(run), C.byref(pct), C.byref(nb))
+    assert rc == 0, "adapter sequence failed or leaked a host buffer / context"
+    out["steps_run"], out["percent"], out["nb"] = run.value, pct.value, nb.value
+    out["ir"] = out["ir"][: run.value]
+    return out
+
+
+def test_macro_defines_every_reference_dispatch_function():
+    text = open(os.path.join(ROOT, "include", "virs_dispatch.hpp")).read()
+    macro = text[text.index("#define VIRS_DEFINE_DISPATCH"):]
+    defined = re.findall(r"(?:void|bool)\s+(\w+)\s*\(SPACE", macro)
+    assert sorted(defined) == sorted(REFERENCE_DISPATCH)
+
+
+def test_driver_library_loads():
+    assert driver() is not None
+
+
+def _room(seed):
+    """Pre-shell voxel flags (0 air, 1 wall, 3 hard source) of a small room."""
+    p = tiny_problem(seed, dims=(28, 20, 16))
+    f = p["flags"].copy()
+    f[f == 2] = 0
+    return f, p
+
+
+def _reference_row(h):
+    # Simulator's material row (simulator.cpp:955-964): RBJ high-pass 440 Hz, Q 0.1
+    dt = 0.5 * h / (343.0 * np.sqrt(3.0))
+    return G.rbj_highpass(np.float32(1.0 / dt), 440.0, 0.1, 1.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("seed", [0, 1])
+def test_simulator_sequence_matches_oracle(gpu, seed):
+    flags, p = _room(seed)
+    nz, ny, nx = flags.shape
+    h = 0.05
+    steps = 90
+    row = _reference_row(h)
+    rcv = (nx // 3, ny // 2, nz // 2)
+    print(f"adapter sequence seed={seed} dims={flags.shape}", flush=True)
+    out = run_sequence(flags, h, row, steps, rcv)
+    shell = G.build_shell(flags)
+    np.testing.assert_array_equal(out["flags"], shell)
+    bidx = G.boundary_list(shell)
+    assert out["nb"] == len(bidx)
+    assert out["steps_run"] == steps
+    ridx = (rcv[2] * ny + rcv[1]) * nx + rcv[0]
+    pb = O.Problem(shell, h, bidx=bidx, coeffs=np.asarray(row, np.float32).astype(np.float64)[None],
+                   rcv_idx=[ridx])
+    ref = pb.run(steps, "f32", want_fields=True)
+    np.testing.assert_array_equal(out["ir"], ref["ir"][0].astype(np.float32))
+    np.testing.assert_array_equal(out["p"], ref["p"].astype(np.float32))
+    np.testing.assert_array_equal(out["q"], ref["q"].astype(np.float32))
+    # p_next after the reference's pointer rotation holds p^{n-2}
+    ref2 = pb.run(steps - 1, "f32", want_fields=True)
+    np.testing.assert_array_equal(out["r"], ref2["q"].astype(np.float32))
+
+
+@pytest.mark.gpu
+def test_rms_rule_stops_where_the_oracle_stops(gpu):
+    flags, p = _room(2)
+    nz, ny, nx = flags.shape
+    h = 0.05
+    steps = 3400
+    row = _reference_row(h)
+    rcv = (nx // 2, ny // 2 + 2, nz // 2)
+    print("adapter RMS stop", flush=True)
+    out = run_sequence(flags, h, row, steps, rcv)
+    shell = G.build_shell(flags)
+    ridx = (rcv[2] * ny + rcv[1]) * nx + rcv[0]
+    pb = O.Problem(shell, h, bidx=G.boundary_list(shell),
+                   coeffs=np.asarray(row, np.float32).astype(np.float64)[None], rcv_idx=[ridx])
+    ir = pb.run(steps, "f32")["ir"][0].astype(np.float32)
+    stop = O.rms_continue_sequence(ir)
+    assert stop > 3000, "the room must ring down inside the run for this test to bite"
+    assert out["steps_run"] == stop + 1
+    np.testing.assert_array_equal(out["ir"], ir[: stop + 1])
+    assert 0.0 < out["percent"]
+
+
+@pytest.mark.gpu
+def test_pressure_sphere_initial_condition(gpu):
+    flags, p = _room(3)
+    nz, ny, nx = flags.shape
+    h = 0.05
+    row = _reference_row(h)
+    sphere = (nx // 2, ny // 2, nz // 2, 3)
+    out = run_sequence(flags, h, row, 0, (1, 1, 1), sphere=sphere, pressure=0.75)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    inside = (x - sphere[0]) ** 2 + (y - sphere[1]) ** 2 + (z - sphere[2]) ** 2 <= sphere[3] ** 2
+    np.testing.assert_array_equal(out["p"], np.where(inside, np.float32(0.75), np.float32(0)))
+    # 40 steps from the sphere: the oracle started from the same p^0
+    steps = 40
+    out = run_sequence(flags, h, row, steps, (nx // 3, ny // 2, nz // 2), sphere=sphere, pressure=0.75)
+    shell = G.build_shell(flags)
+    ridx = (nz // 2 * ny + ny // 2) * nx + nx // 3
+    pb = O.Problem(shell, h, bidx=G.boundary_list(shell),
+                   coeffs=np.asarray(row, np.float32).astype(np.float64)[None], rcv_idx=[ridx],
+                   init_p=np.where(inside, 0.75, 0.0).reshape(-1))
+    ref = pb.run(steps, "f32", want_fields=True)
+    np.testing.assert_array_equal(out["ir"], ref["ir"][0].astype(np.float32))
+    np.testing.assert_array_equal(out["p"], ref["p"].astype(np.float32))

@@ -266,3 +266,106 @@ def test_reference_cpu_step_single_cell():
         lap = (np.float32(0) - np.float32(6) * p) * ih2
         p, q = (np.float32(2) - damp) * p - (np.float32(1) - damp) * q + c2 * lap, p
     assert out[1, 1, 1] == p
+
+
+# --- 16-bit storage runner -------------------------------------------------
+#   fp16 nearest    wrong rounding constant / exponent rebias (numpy is RNE)
+#   bf16 nearest    a truncation or a tie that does not go to even
+#   stochastic      a biased or wrongly masked random add (the mean over
+#                   every random value must reproduce the input exactly)
+#   scale shift     a power-of-two scale applied to the wrong field or step
+
+def _fp16_test_values(rng, n):
+    mags = np.exp(rng.uniform(np.log(1e-9), np.log(7e4), n)).astype(np.float32)
+    x = mags * rng.choice([-1, 1], n).astype(np.float32)
+    # exact ties halfway between neighbouring halfs, normal and subnormal
+    h = rng.integers(1, 0x7bff, 2000, dtype=np.uint16).view(np.float16).astype(np.float64)
+    nxt = (h.astype(np.float16).view(np.uint16) + 1).view(np.float16).astype(np.float64)
+    ties = ((h + nxt) / 2).astype(np.float32)
+    return np.concatenate([x, ties, np.float32([0.0, -0.0, 65504.0, 65519.0, 65520.0, 1e6, np.inf, -np.inf])])
+
+
+def test_fp16_nearest_matches_numpy():
+    x = _fp16_test_values(np.random.default_rng(0), 200000)
+    bits, back = O.round16(x, "fp16")
+    with np.errstate(over="ignore"):
+        want = x.astype(np.float16)
+    np.testing.assert_array_equal(bits, want.view(np.uint16))
+    np.testing.assert_array_equal(back, want.astype(np.float32))
+
+
+def test_fp16_every_pattern_round_trips():
+    pat = np.arange(0x10000, dtype=np.uint32).astype(np.uint16)
+    h = pat.view(np.float16)
+    keep = np.isfinite(h)
+    bits, back = O.round16(h[keep].astype(np.float32), "fp16")
+    np.testing.assert_array_equal(bits, pat[keep])
+    np.testing.assert_array_equal(back, h[keep].astype(np.float32))
+
+
+def test_bf16_nearest_is_the_nearest_value_ties_to_even():
+    rng = np.random.default_rng(1)
+    u = rng.integers(0, 0x7f7f0000, 200000, dtype=np.uint32)
+    u[:2000] = (u[:2000] & 0xffff0000) | 0x8000          # exact ties
+    x = u.view(np.float32) * rng.choice([-1, 1], u.size).astype(np.float32)
+    bits, back = O.round16(x, "bf16")
+    lo = (x.view(np.uint32) & 0xffff0000).view(np.float32).astype(np.float64)
+    hi = ((x.view(np.uint32) & 0xffff0000) + 0x10000).view(np.float32).astype(np.float64)
+    xd = x.astype(np.float64)
+    dlo, dhi = np.abs(xd - lo), np.abs(xd - hi)
+    got = back.astype(np.float64)
+    assert np.all(np.abs(xd - got) <= np.minimum(dlo, dhi))
+    tie = dlo == dhi
+    assert np.all((bits[tie] & 1) == 0)
+    np.testing.assert_array_equal(back.view(np.uint32) & 0xffff, 0)
+
+
+@pytest.mark.parametrize("fmt,nbits", [("fp16", 13), ("bf16", 16)])
+def test_stochastic_rounding_is_exactly_unbiased(fmt, nbits):
+    """Over every value of the random bits, stochastic rounding returns one of
+    the two neighbouring representable values and its mean is the input
+    exactly (normal range: a float32 has exactly `nbits` more mantissa bits)."""
+    rng = np.random.default_rng(2)
+    lo, hi = (2.0 ** -14, 6.0e4) if fmt == "fp16" else (1e-30, 1e30)
+    x = (np.exp(rng.uniform(np.log(lo), np.log(hi), 64)) * rng.choice([-1, 1], 64)).astype(np.float32)
+    r = np.arange(1 << nbits, dtype=np.uint32)
+    for v in x:
+        _, back = O.round16(np.full(r.size, v, np.float32), fmt, r)
+        vals = np.unique(back)
+        assert len(vals) <= 2
+        if fmt == "fp16":
+            near = v.astype(np.float16).astype(np.float32)
+            assert near in vals
+        assert np.sum(back.astype(np.float64)) / r.size == float(v)
+
+
+@pytest.mark.parametrize("fmt", ["fp16", "bf16"])
+@pytest.mark.parametrize("stochastic", [False, True])
+@pytest.mark.parametrize("k", [-7, 5])
+def test_16bit_runner_is_invariant_under_power_of_two_input_scaling(fmt, stochastic, k):
+    """Scaling every input by 2^k must leave the stored patterns unchanged and
+    shift each field exponent by -k: the scale is applied exactly once to
+    each load and store."""
+    p = tiny_problem(40, steps=50)
+    n = np.arange(p["steps"])
+    hard = np.where(n < 10, 10.0, 0.0)
+    outs = []
+    for sc in (1.0, 2.0 ** k):
+        q = dict(p)
+        q["soft_sig"] = p["soft_sig"] * sc
+        pb = oracle_problem(q)
+        pb.hard_sig = hard * sc
+        outs.append(pb.run16(p["steps"], fmt, stochastic, want_fields=True))
+    a, b = outs
+    np.testing.assert_array_equal(b["exp"], a["exp"] - k)
+    np.testing.assert_array_equal(b["ir"], a["ir"] * 2.0 ** k)
+    np.testing.assert_array_equal(b["p"], a["p"] * 2.0 ** k)
+    np.testing.assert_array_equal(b["q"], a["q"] * 2.0 ** k)
+
+
+@pytest.mark.parametrize("fmt,bound", [("fp16", 2e-3), ("bf16", 3e-2)])
+def test_16bit_runner_tracks_the_fp64_scheme(fmt, bound):
+    p = tiny_problem(41, steps=120)
+    ref = oracle_problem(p).run(p["steps"], "f64")["ir"]
+    got = oracle_problem(p).run16(p["steps"], fmt, True)["ir"]
+    assert rel_l2(got, ref) <= bound

@@ -5,6 +5,9 @@
 produces
     virs_amd/lib/libvirs_hip.so   HIP kernels + C ABI   (include/virs_fdtd.h)
     virs_amd/lib/libvirs_host.so  host grid pipeline     (include/virs_host.h)
+and, for the tests, the dispatch adapter (include/virs_dispatch.hpp) compiled
+against a stand-in VectorSpace:
+    tests/adapter/_build/libdispatch_driver.so
 """
 import os
 import subprocess
@@ -50,6 +53,18 @@ def build(verbose=False):
     return hip_out, host_out
 
 
+def build_dispatch_driver(verbose=False):
+    src = os.path.join(ROOT, "tests", "adapter", "dispatch_driver.cpp")
+    out_dir = os.path.join(ROOT, "tests", "adapter", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "libdispatch_driver.so")
+    _run(["g++"] + HOST_FLAGS + ["-Wall", "-Wextra", src, "-o", out + ".tmp", f"-L{LIB}",
+                                 "-lvirs_hip", "-lvirs_host", "-Wl,-rpath,$ORIGIN/../../../virs_amd/lib"],
+         verbose)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_oracle(verbose=False):
     cmd = ["make", "-s", "-C", os.path.join(ROOT, "oracle")]
     _run(cmd, verbose)
@@ -58,4 +73,5 @@ def build_oracle(verbose=False):
 
 if __name__ == "__main__":
     build(verbose=True)
+    build_dispatch_driver(verbose=True)
     build_oracle(verbose=True)

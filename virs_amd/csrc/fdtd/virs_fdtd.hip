@@ -823,17 +823,20 @@ int virs_step(virs_ctx *c, int64_t nsteps, int32_t *keep_going)
         ++c->steps;
     }
     c->counted = c->steps;
-    if (c->es_enable && c->nrcv > 0) {
-        // RMS rule of simulator_gpu.cu:303-330 on receiver 0, in float
+    const unsigned RMS_SIZE = 100, MIN_STEP = 3000;
+    if (c->es_enable && c->nrcv > 0 && c->steps > (long long)MIN_STEP + 1) {
+        // RMS rule of simulator_gpu.cu:303-330 on receiver 0, in float.  No
+        // step up to MIN_STEP can stop the run, so the samples are fetched
+        // (and the stream waited on) only from then on.
+        const long long have = (long long)c->ir0_host.size();
         const long long last = std::min<long long>(c->steps, c->ir_cap);
-        if (last > first) {
-            std::vector<double> tmp((size_t)(last - first));
-            HIPC(c, hipMemcpyAsync(tmp.data(), c->ir_buf + first, sizeof(double) * (last - first),
+        if (last > have) {
+            std::vector<double> tmp((size_t)(last - have));
+            HIPC(c, hipMemcpyAsync(tmp.data(), c->ir_buf + have, sizeof(double) * (last - have),
                                    hipMemcpyDeviceToHost, c->stream));
             HIPC(c, hipStreamSynchronize(c->stream));
             for (double v : tmp) c->ir0_host.push_back((float)v);
         }
-        const unsigned RMS_SIZE = 100, MIN_STEP = 3000;
         for (long long s = first; s < last; ++s) {
             const unsigned step = (unsigned)s;
             if (step > MIN_STEP) {
@@ -901,7 +904,7 @@ int virs_read_receivers(virs_ctx *c, float *out)
 
 int virs_field_exponent(virs_ctx *c, int32_t which, int32_t *e)
 {
-    if (!c || !e || which < 0 || which > 1) return VIRS_ERR_ARG;
+    if (!c || !e || which < 0 || which > 2) return VIRS_ERR_ARG;
     *e = 0;
     if (c->d.precision != VIRS_FP16 && c->d.precision != VIRS_BF16) return VIRS_OK;
     HIPC(c, hipSetDevice(c->d.device));
@@ -912,16 +915,21 @@ int virs_field_exponent(virs_ctx *c, int32_t which, int32_t *e)
     return VIRS_OK;
 }
 
+static const void *field_buffer(const virs_ctx *c, int which)
+{
+    return c->f[which == 0 ? c->ip : which == 1 ? c->iq : c->ir];
+}
+
 int virs_read_field_f64(virs_ctx *c, int32_t which, double *out)
 {
-    if (!c || !out || which < 0 || which > 1) return VIRS_ERR_ARG;
+    if (!c || !out || which < 0 || which > 2) return VIRS_ERR_ARG;
     int e = 0;
     int rc = virs_field_exponent(c, which, &e);
     if (rc) return rc;
     double *tmp = nullptr;
     rc = dalloc(c, &tmp, sizeof(double) * c->N);
     if (rc) return rc;
-    const void *src = c->f[which == 0 ? c->ip : c->iq];
+    const void *src = field_buffer(c, which);
     const int g = 4096;
     switch (c->d.precision) {
     case VIRS_FP16: hipLaunchKernelGGL(k_to_f64<f16>, dim3(g), dim3(256), 0, c->stream, (const f16 *)src, tmp, c->N, e); break;
@@ -939,7 +947,14 @@ int virs_read_field_f64(virs_ctx *c, int32_t which, double *out)
 
 int virs_read_field(virs_ctx *c, int32_t which, float *out)
 {
-    if (!c || !out) return VIRS_ERR_ARG;
+    if (!c || !out || which < 0 || which > 2) return VIRS_ERR_ARG;
+    if (c->d.precision == VIRS_FP32) {
+        HIPC(c, hipSetDevice(c->d.device));
+        HIPC(c, hipMemcpyAsync(out, field_buffer(c, which), sizeof(float) * c->N, hipMemcpyDeviceToHost,
+                               c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        return VIRS_OK;
+    }
     std::vector<double> tmp((size_t)c->N);
     int rc = virs_read_field_f64(c, which, tmp.data());
     if (rc) return rc;

@@ -171,6 +171,15 @@ def boundary_list(flags):
     return idx
 
 
+def rbj_highpass(fs, fc, q, gain=1.0):
+    """RBJ cookbook high-pass row (b0, b1, b2, a1, a2) in float32, computed as
+    src/simulator/biquad.cpp:24-43 does."""
+    out = np.zeros(5, np.float32)
+    _check(_lib.host().virs_rbj_highpass(float(fs), float(fc), float(q), float(gain),
+                                         out.ctypes.data_as(_lib.c_fp)), "rbj high-pass")
+    return out
+
+
 def boundary_materials(flags, wall_mat, bidx):
     L = _lib.host()
     flags = np.ascontiguousarray(flags, np.uint8)
