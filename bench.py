@@ -107,6 +107,50 @@ def cpu_baseline(prep, crop=128, steps=200, warmup=5):
     }
 
 
+def adjoint_cost(steps=4096, ncp=64):
+    """Adjoint gradient on the 256x192x128 shoebox-generator room (3 materials,
+    4 receivers, hidden seeded target coefficients, L2 loss): wall time of
+    the recording forward plus the reverse pass against one plain forward
+    pass, and the FP32 gradient's error against the FP64 adjoint."""
+    from virs_amd import rooms
+    from virs_amd.fdtd import setup
+    sc = rooms.shoebox_grid((256, 192, 128), nmat=3, steps=steps, nrcv=4, seed=7)
+    prep = rooms.prepare(sc)
+    rng = np.random.default_rng(99)
+    target_c = np.array([rooms.random_biquad(rng) for _ in range(3)])
+    out, grads = {}, {}
+    for prec in ("fp64", "fp32"):
+        f = setup(prep, prec)
+        f.set_materials(target_c)
+        f.start(steps)
+        f.step(steps)
+        target = f.receivers(True)
+        f.set_materials(sc.coeffs)
+        f.start(steps)
+        f.step(16)
+        f.synchronize()
+        f.start(steps)
+        t0 = time.perf_counter()
+        f.step(steps)
+        f.synchronize()
+        t_fwd = time.perf_counter() - t0
+        f.start(steps)
+        t0 = time.perf_counter()
+        f.forward_record(steps, ncp)
+        t_rec = time.perf_counter() - t0
+        dl = (2 * (f.receivers(True) - target)).astype(np.float32)
+        t0 = time.perf_counter()
+        grads[prec] = f.adjoint(dl)
+        t_adj = time.perf_counter() - t0
+        out[prec] = {"forward_s": round(t_fwd, 4), "record_s": round(t_rec, 4), "reverse_s": round(t_adj, 4),
+                     "cost_vs_forward": round((t_rec + t_adj) / t_fwd, 3)}
+        f.close()
+    g32, g64 = grads["fp32"], grads["fp64"]
+    err = float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))
+    return {"workload": f"shoebox 256x192x128, 3 materials, 4 receivers, {steps} steps, {ncp} checkpoints",
+            "fp32": out["fp32"], "fp64": out["fp64"], "rel_err_fp32_vs_fp64": float(f"{err:.3e}")}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +162,7 @@ def main():
     ap.add_argument("--fuse", action="store_true", help="run the boundary filter inside the stencil pass")
     ap.add_argument("--vw16", type=int, default=0, help="vector width of the 16-bit variants (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes 5 times (the reference "
                          "point that scales rocprof FETCH_SIZE/WRITE_SIZE to bytes)")
@@ -196,6 +241,8 @@ def main():
                           "dense_ms": round(rv["dense_ms"], 5)}
     if variants:
         out["variants"] = variants
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_cost()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prep)
     print(json.dumps(out), flush=True)

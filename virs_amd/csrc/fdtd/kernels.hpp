@@ -228,6 +228,13 @@ __device__ inline C wave_sum(C v)
     return v;
 }
 
+__device__ inline int wave_max_int(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
 // Cell code, one byte per cell, built on the device from the reference flags:
 //   bits 0-5  neighbour is a wall (flag 1): +x 1, -x 2, +y 4, -y 8, +z 16, -z 32
 //   bits 6-7  0 update (flag 0), 3 update + boundary filter (interior flag 2),
@@ -308,7 +315,10 @@ struct BoundaryState {
     const double *coeffs;      // [nmat][5] b0 b1 b2 a1 a2
     int nmat;
     float kconst;              // (c*dt)/h*0.95f
-    double *trace_d, *trace_y; // [slots][nb], slot = n - trace_base + 2; null = off
+    // forward boundary trace for the adjoint, in the compute type C of the
+    // field (float for FP32, double for FP64): d^n and y^n of every entry,
+    // [slots][nb] with slot = n - trace_base + 2; null = off
+    void *trace_d, *trace_y;
     long long trace_base, trace_slots;
 };
 
@@ -359,8 +369,8 @@ __device__ inline C boundary_update_pre(const BoundaryState &bs, long long e, lo
     if (bs.trace_d) {
         const long long slot = n - bs.trace_base + 2;
         if (slot >= 0 && slot < bs.trace_slots) {
-            bs.trace_d[slot * bs.nb + e] = in;
-            bs.trace_y[slot * bs.nb + e] = yv;
+            ((C *)bs.trace_d)[slot * bs.nb + e] = (C)in;
+            ((C *)bs.trace_y)[slot * bs.nb + e] = (C)yv;
         }
     }
     return (C)((double)r - (double)bs.kconst * (yv * 0.1 + in * 0.9) * (double)0.4f);
@@ -879,9 +889,13 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
         const long long b = p.bidx[e];
         m = bs.bmat ? bs.bmat[e] : 0;
         const double *cf = bs.coeffs + 5 * m;
+        const bool spg = p.sp.w > 0;
         double g = 1.0;
-        sponge_g(p, b, g);
-        double rb = (double)R[b] / g;
+        double rb = (double)R[b];
+        if (spg) {
+            sponge_g(p, b, g);
+            rb = rb / g;
+        }
         const double yb1 = bs.s0[e], yb2 = bs.s1[e], db1 = bs.s2[e], db2 = bs.s3[e];
         rb = rb + db1 - db2;
         if (in_g)
@@ -891,11 +905,12 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
         const double yb = -0.1 * kap * rb - cf[3] * yb1 - cf[4] * yb2;
         const double db = -0.9 * kap * rb + cf[0] * yb + cf[1] * yb1 + cf[2] * yb2;
         const long long sl = n - bs.trace_base + 2;
-        const double dn = bs.trace_d[sl * bs.nb + e];
-        const double x1 = bs.trace_d[(sl - 1) * bs.nb + e];
-        const double x2 = bs.trace_d[(sl - 2) * bs.nb + e];
-        const double y1 = bs.trace_y[(sl - 1) * bs.nb + e];
-        const double y2 = bs.trace_y[(sl - 2) * bs.nb + e];
+        const T *td = (const T *)bs.trace_d, *ty = (const T *)bs.trace_y;
+        const double dn = (double)td[sl * bs.nb + e];
+        const double x1 = (double)td[(sl - 1) * bs.nb + e];
+        const double x2 = (double)td[(sl - 2) * bs.nb + e];
+        const double y1 = (double)ty[(sl - 1) * bs.nb + e];
+        const double y2 = (double)ty[(sl - 2) * bs.nb + e];
         gv[0] = yb * dn;
         gv[1] = yb * x1;
         gv[2] = yb * x2;
@@ -905,7 +920,7 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
         bs.s0[e] = yb;
         bs.s3[e] = db1;
         bs.s2[e] = db;
-        R[b] = (C)(rb * g);
+        R[b] = (C)(spg ? rb * g : rb);
     }
     if (blockIdx.x == 0) {
         const int k = threadIdx.x;
@@ -917,12 +932,28 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
         }
         if (k == 0) p.ctl->nring[p.next_slot] = n - 1;
     }
-    // per-material block sums, fixed reduction order
-    for (int mm = 0; mm < bs.nmat; ++mm) {
+    // per-material block sums, fixed reduction order.  Lanes without an
+    // entry hold m = -1 and zero terms; a wave whose entries share one
+    // material reduces only that material's five sums (the others are the
+    // sums of zeros either way, so the result is the same bit for bit).
+    const int mw = wave_max_int(m);
+    const bool uniform = __ballot(m >= 0 && m != mw) == 0ull;
+    if (uniform) {
+        if (lane < 5 * bs.nmat) part[wave][lane] = 0.0;
+        if (mw >= 0) {
 #pragma unroll
-        for (int c = 0; c < 5; ++c) {
-            const double s = wave_sum(m == mm ? gv[c] : 0.0);
-            if (lane == 0) part[wave][mm * 5 + c] = s;
+            for (int c = 0; c < 5; ++c) {
+                const double s = wave_sum(gv[c]);
+                if (lane == 0) part[wave][mw * 5 + c] = s;
+            }
+        }
+    } else {
+        for (int mm = 0; mm < bs.nmat; ++mm) {
+#pragma unroll
+            for (int c = 0; c < 5; ++c) {
+                const double s = wave_sum(m == mm ? gv[c] : 0.0);
+                if (lane == 0) part[wave][mm * 5 + c] = s;
+            }
         }
     }
     __syncthreads();
@@ -973,15 +1004,16 @@ __global__ void k_ctl_set(StepCtl *ctl, int slot, long long n, int reset)
 
 // Copies the filter state at a segment start into trace slots 0 (step s0-2)
 // and 1 (step s0-1): x2,y2 and x1,y1.
+template <typename C>
 __global__ void k_trace_seed(const double *x1, const double *x2, const double *y1, const double *y2,
-                             double *trace_d, double *trace_y, long long nb)
+                             C *trace_d, C *trace_y, long long nb)
 {
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < nb;
          e += (long long)gridDim.x * blockDim.x) {
-        trace_d[e] = x2[e];
-        trace_y[e] = y2[e];
-        trace_d[nb + e] = x1[e];
-        trace_y[nb + e] = y1[e];
+        trace_d[e] = (C)x2[e];
+        trace_y[e] = (C)y2[e];
+        trace_d[nb + e] = (C)x1[e];
+        trace_y[nb + e] = (C)y1[e];
     }
 }
 

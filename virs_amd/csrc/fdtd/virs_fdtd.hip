@@ -85,9 +85,13 @@ struct virs_ctx {
     long long rec_steps = 0, seg = 0;
     std::vector<void *> cp_field;   // [ncp][2] stored fields
     std::vector<double *> cp_state; // [ncp] 4*nb
-    double *trace_d = nullptr, *trace_y = nullptr;
+    // two boundary-trace buffers (segment k uses buffer k & 1), elements of
+    // the field's compute type; a segment's recomputation on stream2 fills
+    // one while the reverse sweep of the next segment reads the other
+    void *trace_d[2] = {nullptr, nullptr}, *trace_y[2] = {nullptr, nullptr};
     long long trace_slots = 0;
-    long long last_seg_base = -1;   // segment whose trace is already on the device
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
     void *af[3] = {nullptr, nullptr, nullptr};
     double *ast[4] = {nullptr, nullptr, nullptr, nullptr};
     double *gsig = nullptr;
@@ -350,7 +354,7 @@ hipError_t launch_tail_adj(const virs_ctx *c, const TailParams &t)
 
 // One forward step on the context's current rotation; optionally records the
 // boundary trace for the adjoint.
-int launch_step(virs_ctx *c, double *trace_d = nullptr, double *trace_y = nullptr,
+int launch_step(virs_ctx *c, void *trace_d = nullptr, void *trace_y = nullptr,
                 long long trace_base = 0, bool receivers = true, hipEvent_t *ev = nullptr)
 {
     DenseParams p = dense_params(c, c->f[c->ip], c->f[c->iq], c->f[c->ir], c->ctl, c->ip);
@@ -411,6 +415,9 @@ int check_ready(virs_ctx *c)
     HIPC(c, hipSetDevice(c->d.device));
     return VIRS_OK;
 }
+
+// bytes per element of the adjoint's boundary trace (the compute type)
+size_t trace_elem(const virs_ctx *c) { return c->d.precision == VIRS_FP64 ? 8 : 4; }
 
 size_t elem_size(int prec)
 {
@@ -576,10 +583,15 @@ int virs_destroy(virs_ctx *c)
     dfree(c->neg);
     dfree(c->bstart);
     dfree(c->ir_buf);
-    dfree(c->trace_d);
-    dfree(c->trace_y);
+    for (int k = 0; k < 2; ++k) {
+        dfree(c->trace_d[k]);
+        dfree(c->trace_y[k]);
+        if (c->ev_rec[k]) (void)hipEventDestroy(c->ev_rec[k]);
+        if (c->ev_rev[k]) (void)hipEventDestroy(c->ev_rev[k]);
+    }
     dfree(c->gsig);
     dfree(c->grad_part);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return VIRS_OK;
@@ -1018,7 +1030,7 @@ int virs_memory_bytes(const virs_ctx *c, int64_t *bytes)
     b += (long long)sizeof(double) * std::max(1, c->nrcv) * std::max<long long>(1, c->ir_cap);
     b += (long long)c->cp_field.size() * (long long)c->esz * c->N;
     b += (long long)c->cp_state.size() * 32 * c->nb;
-    b += 2 * (long long)sizeof(double) * c->trace_slots * c->nb;
+    b += 4 * (long long)trace_elem(c) * c->trace_slots * c->nb;
     for (auto p : c->af)
         if (p) b += (long long)c->esz * c->N;
     *bytes = b;
@@ -1070,31 +1082,86 @@ int virs_time_steps(virs_ctx *c, int64_t nsteps, double *total_ms, double *dense
 // Forward: snapshots of (p^n, p^{n-1}, filter state) at the start of every
 // segment of `seg` steps; the last segment records its boundary trace
 // (d^n, y^n per entry) directly.  Reverse, segment by segment from the end:
-// restore the snapshot, recompute the segment forward recording the trace,
-// then run the adjoint steps of the segment backwards.  Cost: one forward,
-// one recomputation of all but the last segment, one adjoint sweep.
+// the reverse steps of segment k read the trace of segment k while segment
+// k-1 is recomputed from its snapshot on a second stream, recording its trace
+// into the other buffer.  Events order the two streams: a recomputation
+// starts once the reverse sweep that last read its buffer has finished, and a
+// reverse sweep once its segment's recomputation has.  Cost: one forward,
+// one recomputation of all but the last segment (overlapped with the reverse
+// sweep), one reverse sweep.
 // ---------------------------------------------------------------------------
+
+// Runs the context's launches on stream s for the lifetime of the object.
+struct OnStream {
+    virs_ctx *c;
+    hipStream_t saved;
+    OnStream(virs_ctx *ctx, hipStream_t s) : c(ctx), saved(ctx->stream) { c->stream = s; }
+    ~OnStream() { c->stream = saved; }
+};
 
 static int ensure_trace(virs_ctx *c, long long slots)
 {
-    if (c->trace_slots >= slots && c->trace_d) return VIRS_OK;
-    dfree(c->trace_d);
-    dfree(c->trace_y);
+    if (c->trace_slots >= slots && c->trace_d[0] && c->trace_d[1]) return VIRS_OK;
+    for (int k = 0; k < 2; ++k) {
+        dfree(c->trace_d[k]);
+        dfree(c->trace_y[k]);
+    }
     c->trace_slots = 0;
-    int rc = dalloc(c, &c->trace_d, sizeof(double) * slots * std::max<long long>(1, c->nb));
-    if (!rc) rc = dalloc(c, &c->trace_y, sizeof(double) * slots * std::max<long long>(1, c->nb));
+    const size_t bytes = trace_elem(c) * slots * std::max<long long>(1, c->nb);
+    int rc = VIRS_OK;
+    for (int k = 0; k < 2 && !rc; ++k) {
+        rc = dalloc(c, &c->trace_d[k], bytes);
+        if (!rc) rc = dalloc(c, &c->trace_y[k], bytes);
+    }
     if (rc) return rc;
     c->trace_slots = slots;
     return VIRS_OK;
 }
 
-static int seed_trace(virs_ctx *c)
+static int seed_trace(virs_ctx *c, int buf)
 {
     if (!c->nb) return VIRS_OK;
-    hipLaunchKernelGGL(k_trace_seed, dim3(grid_for(c->nb)), dim3(256), 0, c->stream, c->st[0], c->st[1],
-                       c->st[2], c->st[3], c->trace_d, c->trace_y, c->nb);
+    if (c->d.precision == VIRS_FP64)
+        hipLaunchKernelGGL(k_trace_seed<double>, dim3(grid_for(c->nb)), dim3(256), 0, c->stream, c->st[0],
+                           c->st[1], c->st[2], c->st[3], (double *)c->trace_d[buf],
+                           (double *)c->trace_y[buf], c->nb);
+    else
+        hipLaunchKernelGGL(k_trace_seed<float>, dim3(grid_for(c->nb)), dim3(256), 0, c->stream, c->st[0],
+                           c->st[1], c->st[2], c->st[3], (float *)c->trace_d[buf],
+                           (float *)c->trace_y[buf], c->nb);
     HIPC(c, hipGetLastError());
     return VIRS_OK;
+}
+
+static int save_checkpoint(virs_ctx *c, int k)
+{
+    HIPC(c, hipMemcpyAsync(c->cp_field[2 * k], c->f[c->ip], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(c->cp_field[2 * k + 1], c->f[c->iq], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
+    for (int j = 0; j < 4 && c->nb; ++j)
+        HIPC(c, hipMemcpyAsync(c->cp_state[k] + j * c->nb, c->st[j], sizeof(double) * c->nb,
+                               hipMemcpyDeviceToDevice, c->stream));
+    return VIRS_OK;
+}
+
+// Recomputes segment k from its snapshot on the current launch stream,
+// recording its trace into buffer k & 1.
+static int recompute_segment(virs_ctx *c, int k)
+{
+    const long long s0 = (long long)k * c->seg, s1 = std::min(c->rec_steps, s0 + c->seg);
+    const int buf = k & 1;
+    c->ip = 0;
+    c->iq = 1;
+    c->ir = 2;
+    HIPC(c, hipMemcpyAsync(c->f[c->ip], c->cp_field[2 * k], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(c->f[c->iq], c->cp_field[2 * k + 1], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
+    for (int j = 0; j < 4 && c->nb; ++j)
+        HIPC(c, hipMemcpyAsync(c->st[j], c->cp_state[k] + j * c->nb, sizeof(double) * c->nb,
+                               hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl, 0, s0, 0);
+    HIPC(c, hipGetLastError());
+    int rc = seed_trace(c, buf);
+    for (long long s = s0; s < s1 && !rc; ++s) rc = launch_step(c, c->trace_d[buf], c->trace_y[buf], s0, false);
+    return rc;
 }
 
 int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
@@ -1128,28 +1195,51 @@ int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
     rc = reset_run(c);
     if (rc) return rc;
     const long long last_base = (long long)(ncp - 1) * c->seg;
+    const int last_buf = (ncp - 1) & 1;
     for (long long s = 0; s < nsteps; ++s) {
         if (s % c->seg == 0) {
-            const int k = (int)(s / c->seg);
-            HIPC(c, hipMemcpyAsync(c->cp_field[2 * k], c->f[c->ip], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
-            HIPC(c, hipMemcpyAsync(c->cp_field[2 * k + 1], c->f[c->iq], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
-            for (int j = 0; j < 4 && c->nb; ++j)
-                HIPC(c, hipMemcpyAsync(c->cp_state[k] + j * c->nb, c->st[j], sizeof(double) * c->nb,
-                                       hipMemcpyDeviceToDevice, c->stream));
-            if (s == last_base) {
-                rc = seed_trace(c);
-                if (rc) return rc;
-            }
+            rc = save_checkpoint(c, (int)(s / c->seg));
+            if (!rc && s == last_base) rc = seed_trace(c, last_buf);
+            if (rc) return rc;
         }
-        if (s >= last_base) rc = launch_step(c, c->trace_d, c->trace_y, last_base, true);
+        if (s >= last_base) rc = launch_step(c, c->trace_d[last_buf], c->trace_y[last_buf], last_base, true);
         else rc = launch_step(c);
         if (rc) return rc;
         ++c->steps;
     }
     c->counted = c->steps;
     c->rec_steps = nsteps;
-    c->last_seg_base = last_base;
     HIPC(c, hipStreamSynchronize(c->stream));
+    return VIRS_OK;
+}
+
+// Reverse steps of segment k on the current launch stream.
+static int reverse_segment(virs_ctx *c, int k, int &ap, int &aq, int &ar)
+{
+    const long long T = c->rec_steps;
+    const long long s0 = (long long)k * c->seg, s1 = std::min(T, s0 + c->seg);
+    const int buf = k & 1;
+    for (long long s = s1 - 1; s >= s0; --s) {
+        DenseParams p = dense_params(c, c->af[ap], c->af[aq], c->af[ar], c->ctl_adj, ap);
+        p.nsoft = 0;
+        HIPC(c, launch_dense(c, p, true));
+        TailParams t = tail_params(c, c->af[ap], c->af[aq], c->af[ar], c->ctl_adj, ap, ar);
+        t.bs = boundary_state(c, c->ast);
+        t.bs.trace_d = c->trace_d[buf];
+        t.bs.trace_y = c->trace_y[buf];
+        t.bs.trace_base = s0;
+        t.bs.trace_slots = c->trace_slots;
+        t.ir = nullptr;
+        t.gsig = c->gsig;
+        t.ir_cap = T;
+        t.nsoft = 0;
+        t.grad_part = c->grad_part;
+        HIPC(c, launch_tail_adj(c, t));
+        const int tp = ap;
+        ap = ar;
+        ar = aq;
+        aq = tp;
+    }
     return VIRS_OK;
 }
 
@@ -1160,12 +1250,17 @@ int virs_adjoint(virs_ctx *c, const float *dl_dir, double *grad)
     if (!grad || (c->nrcv && !dl_dir)) return VIRS_ERR_ARG;
     if (c->rec_steps <= 0 || c->ncp <= 0) return fail(c, VIRS_ERR_STATE, "virs_forward_record first");
     const long long T = c->rec_steps;
-    // adjoint buffers
+    // adjoint buffers, second stream and its events
     for (int k = 0; k < 3 && !rc; ++k)
         if (!c->af[k]) rc = dalloc(c, &c->af[k], c->esz * (size_t)c->N);
     for (int k = 0; k < 4 && !rc; ++k)
         if (!c->ast[k]) rc = dalloc(c, &c->ast[k], sizeof(double) * std::max<long long>(1, c->nb));
     if (rc) return rc;
+    if (!c->stream2) HIPC(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    for (int k = 0; k < 2; ++k) {
+        if (!c->ev_rec[k]) HIPC(c, hipEventCreateWithFlags(&c->ev_rec[k], hipEventDisableTiming));
+        if (!c->ev_rev[k]) HIPC(c, hipEventCreateWithFlags(&c->ev_rev[k], hipEventDisableTiming));
+    }
     dfree(c->gsig);
     rc = dalloc(c, &c->gsig, sizeof(double) * std::max(1, c->nrcv) * T);
     if (rc) return rc;
@@ -1188,59 +1283,39 @@ int virs_adjoint(virs_ctx *c, const float *dl_dir, double *grad)
         if (c->nb) HIPC(c, hipMemsetAsync(c->ast[k], 0, sizeof(double) * c->nb, c->stream));
     hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl_adj, 0, T - 1, 1);
     HIPC(c, hipGetLastError());
+    // the second stream starts after everything queued so far
+    HIPC(c, hipStreamSynchronize(c->stream));
     int ap = 0, aq = 1, ar = 2;
-    for (int k = c->ncp - 1; k >= 0; --k) {
-        const long long s0 = (long long)k * c->seg, s1 = std::min(T, s0 + c->seg);
-        if (s0 != c->last_seg_base) {
-            // restore the snapshot into the forward buffers and recompute
-            c->ip = 0;
-            c->iq = 1;
-            c->ir = 2;
-            HIPC(c, hipMemcpyAsync(c->f[c->ip], c->cp_field[2 * k], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
-            HIPC(c, hipMemcpyAsync(c->f[c->iq], c->cp_field[2 * k + 1], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
-            for (int j = 0; j < 4 && c->nb; ++j)
-                HIPC(c, hipMemcpyAsync(c->st[j], c->cp_state[k] + j * c->nb, sizeof(double) * c->nb,
-                                       hipMemcpyDeviceToDevice, c->stream));
-            hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl, 0, s0, 0);
-            HIPC(c, hipGetLastError());
-            rc = seed_trace(c);
+    const int last = c->ncp - 1;
+    if (last >= 1) {
+        OnStream on(c, c->stream2);
+        rc = recompute_segment(c, last - 1);
+        if (!rc) HIPC(c, hipEventRecord(c->ev_rec[(last - 1) & 1], c->stream));
+        if (rc) return rc;
+    }
+    for (int k = last; k >= 0; --k) {
+        if (k != last) HIPC(c, hipStreamWaitEvent(c->stream, c->ev_rec[k & 1], 0));
+        rc = reverse_segment(c, k, ap, aq, ar);
+        if (rc) return rc;
+        HIPC(c, hipEventRecord(c->ev_rev[k & 1], c->stream));
+        if (k - 2 >= 0) {
+            // segment k-2 reuses the buffer the sweep just enqueued reads
+            OnStream on(c, c->stream2);
+            HIPC(c, hipStreamWaitEvent(c->stream, c->ev_rev[k & 1], 0));
+            rc = recompute_segment(c, k - 2);
+            if (!rc) HIPC(c, hipEventRecord(c->ev_rec[(k - 2) & 1], c->stream));
             if (rc) return rc;
-            for (long long s = s0; s < s1; ++s) {
-                rc = launch_step(c, c->trace_d, c->trace_y, s0, false);
-                if (rc) return rc;
-            }
-            c->last_seg_base = s0;
-        }
-        for (long long s = s1 - 1; s >= s0; --s) {
-            DenseParams p = dense_params(c, c->af[ap], c->af[aq], c->af[ar], c->ctl_adj, ap);
-            p.nsoft = 0;
-            HIPC(c, launch_dense(c, p, true));
-            TailParams t = tail_params(c, c->af[ap], c->af[aq], c->af[ar], c->ctl_adj, ap, ar);
-            t.bs = boundary_state(c, c->ast);
-            t.bs.trace_d = c->trace_d;
-            t.bs.trace_y = c->trace_y;
-            t.bs.trace_base = s0;
-            t.bs.trace_slots = c->trace_slots;
-            t.ir = nullptr;
-            t.gsig = c->gsig;
-            t.ir_cap = T;
-            t.nsoft = 0;
-            t.grad_part = c->grad_part;
-            HIPC(c, launch_tail_adj(c, t));
-            const int tp = ap;
-            ap = ar;
-            ar = aq;
-            aq = tp;
         }
     }
     std::vector<double> part((size_t)5 * 8 * gb);
     HIPC(c, hipMemcpyAsync(part.data(), c->grad_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream2));
     for (int i = 0; i < 5 * c->nmat; ++i) grad[i] = 0.0;
     for (long long b = 0; b < gb; ++b)
         for (int i = 0; i < 5 * c->nmat; ++i) grad[i] += part[(size_t)b * 5 * c->nmat + i];
-    // leave the forward state consistent with a finished run
-    c->last_seg_base = -1;
+    // the forward state is left as after a finished run; a new
+    // virs_forward_record is needed before the next adjoint
     c->rec_steps = 0;
     return VIRS_OK;
 }
