@@ -151,6 +151,74 @@ def adjoint_cost(steps=4096, ncp=64):
             "fp32": out["fp32"], "fp64": out["fp64"], "rel_err_fp32_vs_fp64": float(f"{err:.3e}")}
 
 
+def large_room(steps=1000, warmup=20, adj_steps=40000, ncp=200):
+    """The 1536x1024x768 room (1.21 G cells, shoebox generator, 5 materials,
+    8 receivers) with FP16 storage: Gvoxel-updates/s, stencil roofline and
+    device footprint; then the FP32 adjoint on a 512^3 corner crop of it with
+    sqrt(N) checkpoints (adj_steps steps, ncp snapshots)."""
+    from virs_amd import rooms
+    from virs_amd.fdtd import BYTES, FDTD, setup
+    t0 = time.perf_counter()
+    sc = rooms.shoebox_grid((1536, 1024, 768), nmat=5, steps=warmup + steps, nrcv=8, seed=11)
+    prep = rooms.prepare(sc)
+    n = prep.room.n
+    print(f"# large room built in {time.perf_counter() - t0:.1f}s: nb={len(prep.room.bidx)}",
+          file=sys.stderr, flush=True)
+    out = {"workload": "shoebox generator 1536x1024x768, 5 materials, 8 receivers, FP16 storage",
+           "cells": n, "boundary_cells": int(len(prep.room.bidx))}
+    f = setup(prep, "fp16")
+    f.start(warmup + steps)
+    f.step(warmup)
+    f.synchronize()
+    total_ms, dense_ms, bnd_ms = f.time_steps(steps)
+    db = n * (3 * BYTES["fp16"] + 1)
+    out.update({"steps": steps, "value": round(n * steps / (total_ms / 1e3) / 1e9, 3),
+                "unit": "Gvoxel-updates/s", "ms_per_step": round(total_ms / steps, 4),
+                "roofline": {"kernel": "k_dense", "kernel_ms": round(dense_ms, 4),
+                             "achieved": round(db / (dense_ms / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                             "frac": round(db / (dense_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+                "boundary_kernel_ms": round(bnd_ms, 4), "device_bytes": f.memory_bytes(),
+                "ir_finite": bool(np.isfinite(f.receivers()).all())})
+    f.close()
+    print(f"# large room {out['value']} Gvox/s, {out['device_bytes'] / 1e9:.2f} GB", file=sys.stderr, flush=True)
+    # FP32 adjoint on the 512^3 corner crop
+    flags, bidx, bmat = rooms.crop(prep, (512, 512, 512))
+    rng = np.random.default_rng(5)
+    air = np.flatnonzero(flags.reshape(-1) == 0)
+    pts = rng.choice(air, 5, replace=False)
+    g = FDTD((512, 512, 512), sc.h, dt=prep.dt, precision="fp32")
+    g.upload_flags(flags)
+    g.upload_boundary(bidx, bmat)
+    g.set_materials(sc.coeffs)
+    g.set_soft_sources(pts[:1], rooms.gauss_deriv(adj_steps)[None])
+    g.set_receivers(pts[1:])
+    est = ncp * 2 * 4 * 512 ** 3 + ncp * 24 * len(bidx)
+    print(f"# crop adjoint: {adj_steps} steps, {ncp} checkpoints (~{est / 1e9:.0f} GB of snapshots)",
+          file=sys.stderr, flush=True)
+    g.start(adj_steps)
+    t0 = time.perf_counter()
+    g.step(adj_steps)
+    g.synchronize()
+    t_fwd = time.perf_counter() - t0
+    print(f"# crop forward {t_fwd:.2f}s", file=sys.stderr, flush=True)
+    g.start(adj_steps)
+    t0 = time.perf_counter()
+    g.forward_record(adj_steps, ncp)
+    t_rec = time.perf_counter() - t0
+    print(f"# crop recording forward {t_rec:.2f}s", file=sys.stderr, flush=True)
+    dl = rng.standard_normal((4, adj_steps)).astype(np.float32) * 1e-3
+    t0 = time.perf_counter()
+    grad = g.adjoint(dl)
+    t_adj = time.perf_counter() - t0
+    out["adjoint_crop"] = {"workload": f"512^3 corner crop, FP32, {adj_steps} steps, {ncp} checkpoints",
+                           "boundary_cells": int(len(bidx)), "forward_s": round(t_fwd, 3),
+                           "record_s": round(t_rec, 3), "reverse_s": round(t_adj, 3),
+                           "cost_vs_forward": round((t_rec + t_adj) / t_fwd, 3),
+                           "device_bytes": g.memory_bytes(), "grad_finite": bool(np.isfinite(grad).all())}
+    g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,6 +231,10 @@ def main():
     ap.add_argument("--vw16", type=int, default=0, help="vector width of the 16-bit variants (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
+    ap.add_argument("--large", action="store_true",
+                    help="only the 1536x1024x768 FP16 room and the FP32 adjoint on its 512^3 crop")
+    ap.add_argument("--adj-steps", type=int, default=40000, help="steps of the --large crop adjoint")
+    ap.add_argument("--adj-checkpoints", type=int, default=200, help="snapshots of the --large crop adjoint")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes 5 times (the reference "
                          "point that scales rocprof FETCH_SIZE/WRITE_SIZE to bytes)")
@@ -180,6 +252,10 @@ def main():
         print("bench.py runs on one GPU only: the FDTD path is measured on a single MI355X "
               "(multi-GPU z-slab sharding is designed, not built)", file=sys.stderr)
         sys.exit(2)
+
+    if args.large:
+        print(json.dumps(large_room(args.steps, args.warmup, args.adj_steps, args.adj_checkpoints)), flush=True)
+        return
 
     from virs_amd import rooms
     t0 = time.perf_counter()

@@ -311,7 +311,15 @@ __device__ inline int edge_dist(int x, int y, int z, int nx, int ny, int nz)
 struct BoundaryState {
     long long nb;
     const uint8_t *bmat;       // may be null: row 0
-    double *s0, *s1, *s2, *s3; // forward: x1 x2 y1 y2; reverse: ybar1 ybar2 dbar1 dbar2
+    // Filter state as two-slot rings indexed by step parity: on entry to
+    // step n, slot n&1 holds the value of step n-2 and slot (n+1)&1 that of
+    // step n-1; step n overwrites slot n&1, so a step writes one value per
+    // ring instead of shifting the state.  Forward: xs = filter inputs d in
+    // the field's compute type C (d is a difference of two C values, so C
+    // holds it exactly), ys = outputs y in FP64.  Reverse: xs = dbar,
+    // ys = ybar, both FP64.
+    void *xs[2];
+    double *ys[2];
     const double *coeffs;      // [nmat][5] b0 b1 b2 a1 a2
     int nmat;
     float kconst;              // (c*dt)/h*0.95f
@@ -350,6 +358,19 @@ struct DenseParams {
     BoundaryState bs;
 };
 
+// Filter taps of entry e before step n: x1 = d^{n-1}, x2 = d^{n-2},
+// y1 = y^{n-1}, y2 = y^{n-2}.
+template <typename C>
+__device__ inline void load_taps(const BoundaryState &bs, long long e, long long n, double &x1, double &x2,
+                                 double &y1, double &y2)
+{
+    const int s = (int)(n & 1);
+    x1 = (double)((const C *)bs.xs[s ^ 1])[e];
+    x2 = (double)((const C *)bs.xs[s])[e];
+    y1 = bs.ys[s ^ 1][e];
+    y2 = bs.ys[s][e];
+}
+
 // DF1 biquad step and boundary correction for list entry e (simulator_gpu.cu:126-148):
 //   y = b0 d + b1 x1 + b2 x2 - a1 y1 - a2 y2   (FP64 state)
 //   R = R - K (0.1 y + 0.9 d) 0.4f             (evaluated in double)
@@ -362,10 +383,8 @@ __device__ inline C boundary_update_pre(const BoundaryState &bs, long long e, lo
     const double *cf = bs.coeffs + 5 * m;
     const double in = (double)dif;
     const double yv = cf[0] * in + cf[1] * x1 + cf[2] * x2 - cf[3] * y1 - cf[4] * y2;
-    bs.s1[e] = x1;
-    bs.s0[e] = in;
-    bs.s3[e] = y1;
-    bs.s2[e] = yv;
+    ((C *)bs.xs[n & 1])[e] = dif;
+    bs.ys[n & 1][e] = yv;
     if (bs.trace_d) {
         const long long slot = n - bs.trace_base + 2;
         if (slot >= 0 && slot < bs.trace_slots) {
@@ -379,7 +398,9 @@ __device__ inline C boundary_update_pre(const BoundaryState &bs, long long e, lo
 template <typename C>
 __device__ inline C boundary_update(const BoundaryState &bs, long long e, long long n, C dif, C r)
 {
-    return boundary_update_pre<C>(bs, e, n, dif, r, bs.s0[e], bs.s1[e], bs.s2[e], bs.s3[e]);
+    double x1, x2, y1, y2;
+    load_taps<C>(bs, e, n, x1, x2, y1, y2);
+    return boundary_update_pre<C>(bs, e, n, dif, r, x1, x2, y1, y2);
 }
 
 // Leapfrog 7-point update, simulator_gpu.cu:38-103 with a graded sponge:
@@ -703,10 +724,7 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
                     pv[u] = S::ld(P[bv[u]]);
                     qv2[u] = S::ld(Q[bv[u]]);
                     rv[u] = S::ld(R[bv[u]]);
-                    x1[u] = p.bs.s0[ev[u]];
-                    x2[u] = p.bs.s1[ev[u]];
-                    y1[u] = p.bs.s2[ev[u]];
-                    y2[u] = p.bs.s3[ev[u]];
+                    load_taps<C>(p.bs, ev[u], n, x1[u], x2[u], y1[u], y2[u]);
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -869,8 +887,8 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
 //   ybar   = -0.1 kap rbar - a1 ybar1 - a2 ybar2          kap = K*0.4f
 //   dbar   = -0.9 kap rbar + b0 ybar + b1 ybar1 + b2 ybar2
 //   dL/d(b0,b1,b2,a1,a2) += ybar * (d^n, d^{n-1}, d^{n-2}, -y^{n-1}, -y^{n-2})
-// where d and y come from the forward trace of this segment.  State arrays:
-// s0 ybar1, s1 ybar2, s2 dbar1, s3 dbar2.  Per-block gradient partials are
+// where d and y come from the forward trace of this segment.  The state
+// rings of BoundaryState hold ybar (ys) and dbar (xs).  Per-block gradient partials are
 // summed in a fixed order, so the result is deterministic.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
@@ -896,7 +914,9 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
             sponge_g(p, b, g);
             rb = rb / g;
         }
-        const double yb1 = bs.s0[e], yb2 = bs.s1[e], db1 = bs.s2[e], db2 = bs.s3[e];
+        const int sp = (int)(n & 1);
+        const double *dbr0 = (const double *)bs.xs[sp ^ 1], *dbr1 = (const double *)bs.xs[sp];
+        const double yb1 = bs.ys[sp ^ 1][e], yb2 = bs.ys[sp][e], db1 = dbr0[e], db2 = dbr1[e];
         rb = rb + db1 - db2;
         if (in_g)
             for (int k = 0; k < p.nrcv; ++k)
@@ -916,10 +936,8 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
         gv[2] = yb * x2;
         gv[3] = -yb * y1;
         gv[4] = -yb * y2;
-        bs.s1[e] = yb1;
-        bs.s0[e] = yb;
-        bs.s3[e] = db1;
-        bs.s2[e] = db;
+        bs.ys[sp][e] = yb;
+        ((double *)bs.xs[sp])[e] = db;
         R[b] = (C)(spg ? rb * g : rb);
     }
     if (blockIdx.x == 0) {
@@ -1005,14 +1023,14 @@ __global__ void k_ctl_set(StepCtl *ctl, int slot, long long n, int reset)
 // Copies the filter state at a segment start into trace slots 0 (step s0-2)
 // and 1 (step s0-1): x2,y2 and x1,y1.
 template <typename C>
-__global__ void k_trace_seed(const double *x1, const double *x2, const double *y1, const double *y2,
+__global__ void k_trace_seed(const C *x1, const C *x2, const double *y1, const double *y2,
                              C *trace_d, C *trace_y, long long nb)
 {
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < nb;
          e += (long long)gridDim.x * blockDim.x) {
-        trace_d[e] = (C)x2[e];
+        trace_d[e] = x2[e];
         trace_y[e] = (C)y2[e];
-        trace_d[nb + e] = (C)x1[e];
+        trace_d[nb + e] = x1[e];
         trace_y[nb + e] = (C)y1[e];
     }
 }

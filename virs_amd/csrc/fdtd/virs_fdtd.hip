@@ -43,7 +43,9 @@ struct virs_ctx {
     long long nb = 0;
     uint32_t *bidx = nullptr;
     uint8_t *bmat = nullptr;
-    double *st[4] = {nullptr, nullptr, nullptr, nullptr};
+    // filter state rings (BoundaryState::xs, ys): st[0..1] filter inputs in
+    // the compute type (4 or 8 bytes), st[2..3] outputs in FP64
+    void *st[4] = {nullptr, nullptr, nullptr, nullptr};
     std::vector<uint32_t> h_bidx;
     std::vector<uint8_t> h_bmat;
 
@@ -84,7 +86,7 @@ struct virs_ctx {
     int ncp = 0;
     long long rec_steps = 0, seg = 0;
     std::vector<void *> cp_field;   // [ncp][2] stored fields
-    std::vector<double *> cp_state; // [ncp] 4*nb
+    std::vector<uint8_t *> cp_state; // [ncp] the four state rings back to back
     // two boundary-trace buffers (segment k uses buffer k & 1), elements of
     // the field's compute type; a segment's recomputation on stream2 fills
     // one while the reverse sweep of the next segment reads the other
@@ -93,7 +95,7 @@ struct virs_ctx {
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_rec[2] = {nullptr, nullptr}, ev_rev[2] = {nullptr, nullptr};
     void *af[3] = {nullptr, nullptr, nullptr};
-    double *ast[4] = {nullptr, nullptr, nullptr, nullptr};
+    void *ast[4] = {nullptr, nullptr, nullptr, nullptr}; // reverse rings: dbar x2, ybar x2 (FP64)
     double *gsig = nullptr;
     double *grad_part = nullptr;
     long long grad_blocks = 0;
@@ -179,15 +181,15 @@ int upload_points(virs_ctx *c)
     return VIRS_OK;
 }
 
-BoundaryState boundary_state(const virs_ctx *c, double *const *st)
+BoundaryState boundary_state(const virs_ctx *c, void *const *st)
 {
     BoundaryState b{};
     b.nb = c->nb;
     b.bmat = c->bmat;
-    b.s0 = st[0];
-    b.s1 = st[1];
-    b.s2 = st[2];
-    b.s3 = st[3];
+    b.xs[0] = st[0];
+    b.xs[1] = st[1];
+    b.ys[0] = (double *)st[2];
+    b.ys[1] = (double *)st[3];
     b.coeffs = c->coeffs;
     b.nmat = c->nmat;
     b.kconst = c->kconst;
@@ -416,8 +418,15 @@ int check_ready(virs_ctx *c)
     return VIRS_OK;
 }
 
-// bytes per element of the adjoint's boundary trace (the compute type)
-size_t trace_elem(const virs_ctx *c) { return c->d.precision == VIRS_FP64 ? 8 : 4; }
+// bytes of the compute type C of a field (float, or double for FP64): the
+// element of the filter-input rings and of the adjoint trace
+size_t compute_bytes(const virs_ctx *c) { return c->d.precision == VIRS_FP64 ? 8 : 4; }
+
+// bytes of filter-state ring k (0, 1: inputs in the compute type; 2, 3: FP64 outputs)
+size_t state_bytes(const virs_ctx *c, int k)
+{
+    return (k < 2 ? compute_bytes(c) : sizeof(double)) * (size_t)std::max<long long>(1, c->nb);
+}
 
 size_t elem_size(int prec)
 {
@@ -441,7 +450,7 @@ int reset_run(virs_ctx *c)
     int rc = zero_fields(c, c->f);
     if (rc) return rc;
     for (int k = 0; k < 4; ++k)
-        if (c->st[k] && c->nb) HIPC(c, hipMemsetAsync(c->st[k], 0, sizeof(double) * c->nb, c->stream));
+        if (c->st[k] && c->nb) HIPC(c, hipMemsetAsync(c->st[k], 0, state_bytes(c, k), c->stream));
     hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl, 0, 0LL, 1);
     HIPC(c, hipGetLastError());
     c->ip = 0;
@@ -670,13 +679,13 @@ int virs_upload_boundary(virs_ctx *c, const uint32_t *idx, const uint8_t *mat, i
     if (mat) c->h_bmat.assign(mat, mat + nb);
     else c->h_bmat.clear();
     int rc = dalloc(c, &c->bidx, sizeof(uint32_t) * nb);
-    for (int k = 0; k < 4 && !rc; ++k) rc = dalloc(c, &c->st[k], sizeof(double) * nb);
+    for (int k = 0; k < 4 && !rc; ++k) rc = dalloc(c, &c->st[k], state_bytes(c, k));
     if (!rc && mat) rc = dalloc(c, &c->bmat, (size_t)nb);
     if (rc) return rc;
     if (nb) HIPC(c, hipMemcpy(c->bidx, idx, sizeof(uint32_t) * nb, hipMemcpyHostToDevice));
     if (nb && mat) HIPC(c, hipMemcpy(c->bmat, mat, (size_t)nb, hipMemcpyHostToDevice));
     for (int k = 0; k < 4; ++k)
-        if (nb) HIPC(c, hipMemset(c->st[k], 0, sizeof(double) * nb));
+        if (nb) HIPC(c, hipMemset(c->st[k], 0, state_bytes(c, k)));
     drop_graphs(c);
     rc = refresh_fusion(c);
     if (rc) return rc;
@@ -1026,11 +1035,11 @@ int virs_memory_bytes(const virs_ctx *c, int64_t *bytes)
     if (!c || !bytes) return VIRS_ERR_ARG;
     long long b = 3 * (long long)c->esz * c->N + c->N; // fields + cell codes
     if (c->normals) b += c->N;
-    b += c->nb * (4 + 32 + (c->bmat ? 1 : 0));
+    b += c->nb * (4 + 2 * (long long)compute_bytes(c) + 16 + (c->bmat ? 1 : 0));
     b += (long long)sizeof(double) * std::max(1, c->nrcv) * std::max<long long>(1, c->ir_cap);
     b += (long long)c->cp_field.size() * (long long)c->esz * c->N;
-    b += (long long)c->cp_state.size() * 32 * c->nb;
-    b += 4 * (long long)trace_elem(c) * c->trace_slots * c->nb;
+    b += (long long)c->cp_state.size() * (2 * (long long)compute_bytes(c) + 16) * c->nb;
+    b += 4 * (long long)compute_bytes(c) * c->trace_slots * c->nb;
     for (auto p : c->af)
         if (p) b += (long long)c->esz * c->N;
     *bytes = b;
@@ -1107,7 +1116,7 @@ static int ensure_trace(virs_ctx *c, long long slots)
         dfree(c->trace_y[k]);
     }
     c->trace_slots = 0;
-    const size_t bytes = trace_elem(c) * slots * std::max<long long>(1, c->nb);
+    const size_t bytes = compute_bytes(c) * slots * std::max<long long>(1, c->nb);
     int rc = VIRS_OK;
     for (int k = 0; k < 2 && !rc; ++k) {
         rc = dalloc(c, &c->trace_d[k], bytes);
@@ -1118,17 +1127,20 @@ static int ensure_trace(virs_ctx *c, long long slots)
     return VIRS_OK;
 }
 
-static int seed_trace(virs_ctx *c, int buf)
+// Trace slots 0 and 1 of buffer buf from the filter state before step s0.
+static int seed_trace(virs_ctx *c, int buf, long long s0)
 {
     if (!c->nb) return VIRS_OK;
+    const int a = (int)(s0 & 1), b = a ^ 1; // ring slots of steps s0-2 and s0-1
+    const double *y1 = (const double *)c->st[2 + b], *y2 = (const double *)c->st[2 + a];
     if (c->d.precision == VIRS_FP64)
-        hipLaunchKernelGGL(k_trace_seed<double>, dim3(grid_for(c->nb)), dim3(256), 0, c->stream, c->st[0],
-                           c->st[1], c->st[2], c->st[3], (double *)c->trace_d[buf],
-                           (double *)c->trace_y[buf], c->nb);
+        hipLaunchKernelGGL(k_trace_seed<double>, dim3(grid_for(c->nb)), dim3(256), 0, c->stream,
+                           (const double *)c->st[b], (const double *)c->st[a], y1, y2,
+                           (double *)c->trace_d[buf], (double *)c->trace_y[buf], c->nb);
     else
-        hipLaunchKernelGGL(k_trace_seed<float>, dim3(grid_for(c->nb)), dim3(256), 0, c->stream, c->st[0],
-                           c->st[1], c->st[2], c->st[3], (float *)c->trace_d[buf],
-                           (float *)c->trace_y[buf], c->nb);
+        hipLaunchKernelGGL(k_trace_seed<float>, dim3(grid_for(c->nb)), dim3(256), 0, c->stream,
+                           (const float *)c->st[b], (const float *)c->st[a], y1, y2,
+                           (float *)c->trace_d[buf], (float *)c->trace_y[buf], c->nb);
     HIPC(c, hipGetLastError());
     return VIRS_OK;
 }
@@ -1137,9 +1149,12 @@ static int save_checkpoint(virs_ctx *c, int k)
 {
     HIPC(c, hipMemcpyAsync(c->cp_field[2 * k], c->f[c->ip], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
     HIPC(c, hipMemcpyAsync(c->cp_field[2 * k + 1], c->f[c->iq], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
-    for (int j = 0; j < 4 && c->nb; ++j)
-        HIPC(c, hipMemcpyAsync(c->cp_state[k] + j * c->nb, c->st[j], sizeof(double) * c->nb,
-                               hipMemcpyDeviceToDevice, c->stream));
+    size_t off = 0;
+    for (int j = 0; j < 4 && c->nb; ++j) {
+        HIPC(c, hipMemcpyAsync(c->cp_state[k] + off, c->st[j], state_bytes(c, j), hipMemcpyDeviceToDevice,
+                               c->stream));
+        off += state_bytes(c, j);
+    }
     return VIRS_OK;
 }
 
@@ -1154,12 +1169,15 @@ static int recompute_segment(virs_ctx *c, int k)
     c->ir = 2;
     HIPC(c, hipMemcpyAsync(c->f[c->ip], c->cp_field[2 * k], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
     HIPC(c, hipMemcpyAsync(c->f[c->iq], c->cp_field[2 * k + 1], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
-    for (int j = 0; j < 4 && c->nb; ++j)
-        HIPC(c, hipMemcpyAsync(c->st[j], c->cp_state[k] + j * c->nb, sizeof(double) * c->nb,
-                               hipMemcpyDeviceToDevice, c->stream));
+    size_t off = 0;
+    for (int j = 0; j < 4 && c->nb; ++j) {
+        HIPC(c, hipMemcpyAsync(c->st[j], c->cp_state[k] + off, state_bytes(c, j), hipMemcpyDeviceToDevice,
+                               c->stream));
+        off += state_bytes(c, j);
+    }
     hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl, 0, s0, 0);
     HIPC(c, hipGetLastError());
-    int rc = seed_trace(c, buf);
+    int rc = seed_trace(c, buf, s0);
     for (long long s = s0; s < s1 && !rc; ++s) rc = launch_step(c, c->trace_d[buf], c->trace_y[buf], s0, false);
     return rc;
 }
@@ -1185,7 +1203,7 @@ int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
         for (int k = 0; k < ncp && !rc; ++k) {
             rc = dalloc(c, &c->cp_field[2 * k], c->esz * (size_t)c->N);
             if (!rc) rc = dalloc(c, &c->cp_field[2 * k + 1], c->esz * (size_t)c->N);
-            if (!rc) rc = dalloc(c, &c->cp_state[k], sizeof(double) * 4 * std::max<long long>(1, c->nb));
+            if (!rc) rc = dalloc(c, &c->cp_state[k], state_bytes(c, 0) * 2 + state_bytes(c, 2) * 2);
         }
         if (rc) return rc;
     }
@@ -1199,7 +1217,7 @@ int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
     for (long long s = 0; s < nsteps; ++s) {
         if (s % c->seg == 0) {
             rc = save_checkpoint(c, (int)(s / c->seg));
-            if (!rc && s == last_base) rc = seed_trace(c, last_buf);
+            if (!rc && s == last_base) rc = seed_trace(c, last_buf, s);
             if (rc) return rc;
         }
         if (s >= last_base) rc = launch_step(c, c->trace_d[last_buf], c->trace_y[last_buf], last_base, true);
