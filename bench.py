@@ -8,8 +8,8 @@ and 8 receivers, with fields already resident in HBM.  Prints one JSON line.
 
 roofline: the stencil launch's algorithmic bytes N*(3s+1) (read p^n and
 p^{n-1}, write p^{n+1}, read one code byte per cell) divided by its mean
-launch time, taken with HIP events around every launch of the timed region
-on the library's own stream.
+launch time, taken with HIP events around the launches of about 64 steps
+spread over the timed region, on the library's own stream.
 cpu_baseline: the CPU oracle (the same scheme, OpenMP over z) on a 128^3
 corner crop of the same room, on this host's cores.
 """

@@ -388,6 +388,10 @@ int oracle_adjoint_f64(const oracle_desc *d, int64_t nsteps, const double *dL_di
  *   - storage rounding is nearest-even or stochastic: 16 bits of a
  *     counter-based hash of (cell, step, salt) are added below the 16-bit
  *     format's last place before truncation (mix32 below);
+ *   - walls and border cells hold zero, so the stencil sums the six
+ *     neighbours as stored and weights the centre by 6 - (wall neighbours),
+ *     which is the reference's substitution V(wall) = pc in exact arithmetic
+ *     (the FP32 sum order differs from oracle_run_f32's);
  *   - the boundary list runs after the stencil on the stored field, soft
  *     sources after the boundary correction on its stored result.
  * ------------------------------------------------------------------------ */
@@ -399,7 +403,7 @@ static uint32_t mix32(uint64_t idx, int64_t n, uint32_t salt)
     const uint64_t pair = idx & ~(uint64_t)1;
     uint32_t h = (uint32_t)pair * 0x9E3779B1u + ((uint32_t)(pair >> 32) + (uint32_t)n) * 0x85EBCA77u + salt;
     h ^= h >> 16;
-    h *= 0x2C1B3C6Du;
+    h = (h & 0xffffffu) * 0x1B3C6Du; /* 24-bit multiply */
     h ^= h >> 15;
     return (idx & 1) ? (h >> 16) : (h & 0xffffu);
 }
@@ -572,19 +576,19 @@ int oracle_run_s16(const oracle_desc *d, int bf16, int sr, int64_t nsteps, doubl
                     if (!border && f == 3) {
                         o = hard;
                     } else if (!border && (f == 0 || f == 2)) {
+                        /* walls and border cells hold zero in 16-bit storage: the
+                         * plain neighbour sum with centre weight kw = 6 - walls
+                         * is the reference's rule V(wall) = pc */
                         const float pc = ld16(&S, P[i]) * sP;
-                        const float v0 = d->flags[i + 1] == 1 ? pc : ld16(&S, P[i + 1]) * sP;
-                        const float v1 = d->flags[i - 1] == 1 ? pc : ld16(&S, P[i - 1]) * sP;
-                        const float v2 = d->flags[i + nx] == 1 ? pc : ld16(&S, P[i + nx]) * sP;
-                        const float v3 = d->flags[i - nx] == 1 ? pc : ld16(&S, P[i - nx]) * sP;
-                        const float v4 = d->flags[i + sxy] == 1 ? pc : ld16(&S, P[i + sxy]) * sP;
-                        const float v5 = d->flags[i - sxy] == 1 ? pc : ld16(&S, P[i - sxy]) * sP;
-                        float sum = v0 + v1;
-                        sum = sum + v2;
-                        sum = sum + v3;
-                        sum = sum + v4;
-                        sum = sum + v5;
-                        const float lap = (sum - 6.f * pc) * ih2;
+                        float sum = ld16(&S, P[i + 1]) * sP + ld16(&S, P[i - 1]) * sP;
+                        sum = sum + ld16(&S, P[i + nx]) * sP;
+                        sum = sum + ld16(&S, P[i - nx]) * sP;
+                        sum = sum + ld16(&S, P[i + sxy]) * sP;
+                        sum = sum + ld16(&S, P[i - sxy]) * sP;
+                        const int walls = (d->flags[i + 1] == 1) + (d->flags[i - 1] == 1) +
+                                          (d->flags[i + nx] == 1) + (d->flags[i - nx] == 1) +
+                                          (d->flags[i + sxy] == 1) + (d->flags[i - sxy] == 1);
+                        const float lap = (sum - (float)(6 - walls) * pc) * ih2;
                         float a = 2.f, b = 1.f, g = 1.f;
                         if (d->sponge_w > 0) {
                             const int dd = sponge_dist(d, x, y, z);

@@ -10,7 +10,8 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_DIR = os.path.join(_HERE, "lib")
+# VIRS_LIB_DIR selects another build of the same libraries (tuning variants)
+LIB_DIR = os.environ.get("VIRS_LIB_DIR") or os.path.join(_HERE, "lib")
 
 _hip = None
 _host = None

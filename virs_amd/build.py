@@ -40,15 +40,17 @@ def _run(cmd, verbose):
         raise RuntimeError(f"build failed: {' '.join(cmd)}")
 
 
-def build(verbose=False):
-    os.makedirs(LIB, exist_ok=True)
+def build(verbose=False, out_dir=None, defines=()):
+    """out_dir / defines build a variant (e.g. VIRS_TAIL_BATCH=2) elsewhere."""
+    lib = out_dir or LIB
+    os.makedirs(lib, exist_ok=True)
     hip_src = os.path.join(PKG, "csrc", "fdtd", "virs_fdtd.hip")
     host_src = os.path.join(PKG, "csrc", "host", "virs_host.cpp")
-    hip_out = os.path.join(LIB, "libvirs_hip.so")
-    host_out = os.path.join(LIB, "libvirs_host.so")
+    hip_out = os.path.join(lib, "libvirs_hip.so")
+    host_out = os.path.join(lib, "libvirs_host.so")
     _run(["g++"] + HOST_FLAGS + [host_src, "-o", host_out + ".tmp"], verbose)
     os.replace(host_out + ".tmp", host_out)
-    _run([HIPCC] + HIP_FLAGS + [hip_src, "-o", hip_out + ".tmp"], verbose)
+    _run([HIPCC] + HIP_FLAGS + [f"-D{d}" for d in defines] + [hip_src, "-o", hip_out + ".tmp"], verbose)
     os.replace(hip_out + ".tmp", hip_out)
     return hip_out, host_out
 

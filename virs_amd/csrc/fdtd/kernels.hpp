@@ -91,19 +91,21 @@ struct Store<bf16> {
 // (cell pair, step, salt) gives 16 random bits per cell (one 32-bit hash
 // serves the cells 2k and 2k+1: low half, high half).  They are masked to
 // the bits below the format's last place (13 for FP16, 16 for BF16), added
-// to the float's bit pattern, the low bits cleared, and the result converted
-// toward zero.  In the normal range that is exactly stochastic rounding:
-// E[stored] = value, so increments smaller than half an ulp are not lost
-// step after step; in FP16's subnormal range the conversion truncates.
+// to the float's bit pattern, and the result converted toward zero, which
+// drops those low bits.  In the normal range that is exactly stochastic
+// rounding: E[stored] = value, so increments smaller than half an ulp are
+// not lost step after step; in FP16's subnormal range the conversion
+// truncates.  The mixing multiply is the full-rate 24-bit one (the xor-shift
+// before it folds the top byte into the low bits).
+constexpr uint32_t kHashK1 = 0x9E3779B1u, kHashK2 = 0x1B3C6Du, kHashK3 = 0x85EBCA77u;
 __device__ inline uint32_t hash_pair(uint32_t lo_k1, uint32_t hk)
 {
     uint32_t h = lo_k1 + hk;
     h ^= h >> 16;
-    h *= 0x2C1B3C6Du;
+    h = __umul24(h, kHashK2);
     h ^= h >> 15;
     return h;
 }
-constexpr uint32_t kHashK1 = 0x9E3779B1u, kHashK3 = 0x85EBCA77u;
 // hk term of the hash for cells idx (any of a pair) at step n
 __device__ inline uint32_t hash_hk(unsigned long long idx, long long n, uint32_t salt)
 {
@@ -118,16 +120,16 @@ __device__ inline uint32_t mix16(unsigned long long idx, long long n, uint32_t s
 
 constexpr uint32_t kSaltDense = 0u, kSaltBoundary = 0x68E31DA4u, kSaltSource = 0xB5297A4Du;
 
-// Bit pattern of x stochastically rounded (random bits r16) to the 16-bit
-// format T, still as a float whose low bits are clear.
+// x with the random bits r16 added below the 16-bit format's last place;
+// converting the result toward zero (cvt2_truncated) rounds x stochastically.
 template <typename T>
 __device__ inline float sr_bits(float x, uint32_t r16)
 {
     constexpr uint32_t mask = __is_same(T, bf16) ? 0xffffu : 0x1fffu;
-    return __uint_as_float((__float_as_uint(x) + (r16 & mask)) & ~mask);
+    return __uint_as_float(__float_as_uint(x) + (r16 & mask));
 }
 
-// Converts two floats whose low bits are already clear to T.
+// Converts two floats to T rounding toward zero.
 template <typename T>
 __device__ inline void cvt2_truncated(float a, float b, T &ta, T &tb)
 {
@@ -275,6 +277,46 @@ __global__ void k_build_code(const uint8_t *__restrict__ flags, uint8_t *__restr
     }
 }
 
+// Cell code of the 16-bit formats.  In 16-bit storage every wall and border
+// cell holds zero, so a wall neighbour contributes nothing to the plain sum of
+// the six neighbours and the reference's substitution V(wall) = pc becomes a
+// smaller centre weight: lap = (sum of neighbours - kw pc) ih2 with
+// kw = 6 - (wall neighbours).  Byte: kw for cells that update, kCode16Zero
+// for walls and border cells, kCode16Hard for hard-source cells.
+constexpr uint8_t kCode16Zero = 0x40, kCode16Hard = 0x80;
+__global__ void k_build_code16(const uint8_t *__restrict__ flags, uint8_t *__restrict__ code, int nx,
+                               int ny, int nz)
+{
+    const long long sxy = (long long)nx * ny, n = sxy * nz;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(i / sxy);
+        const long long r = i - (long long)z * sxy;
+        const int y = (int)(r / nx);
+        const int x = (int)(r - (long long)y * nx);
+        uint8_t c;
+        if (x == 0 || y == 0 || z == 0 || x == nx - 1 || y == ny - 1 || z == nz - 1 || flags[i] == 1) {
+            c = kCode16Zero;
+        } else if (flags[i] == 3) {
+            c = kCode16Hard;
+        } else {
+            const int walls = (flags[i + 1] == 1) + (flags[i - 1] == 1) + (flags[i + nx] == 1) +
+                              (flags[i - nx] == 1) + (flags[i + sxy] == 1) + (flags[i - sxy] == 1);
+            c = (uint8_t)(6 - walls);
+        }
+        code[i] = c;
+    }
+}
+
+// Zeroes the cells a 16-bit code marks as walls or border.
+template <typename T>
+__global__ void k_zero_walls(T *__restrict__ f, const uint8_t *__restrict__ code, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        if (code[i] & kCode16Zero) f[i] = Store<T>::st(0.f);
+}
+
 // All-ones if bit `bit` of c is set, else zero (one signed bit-field extract).
 __device__ inline uint32_t bit_mask(uint32_t c, int bit) { return (uint32_t)__builtin_amdgcn_sbfe((int)c, bit, 1); }
 // Bitwise select, a where m is set, else b, as one v_bfi_b32 (the compiler
@@ -415,12 +457,17 @@ __device__ inline C boundary_update(const BoundaryState &bs, long long e, long l
 // cell type, without branches.
 // 16-bit storage: arithmetic runs in p^n's stored units (value * 2^eP);
 // power-of-two scaling commutes exactly with every rounding step, so the
-// results equal a computation in physical units bit for bit.
+// results equal a computation in physical units bit for bit.  Walls hold
+// zero and the code carries the centre weight kw (k_build_code16); SR
+// selects stochastic storage rounding.
 // FUSED: the boundary biquad of the block's list entries runs at the end of
 // the same launch.  ADJ: the reverse sweep applies the same operator to the adjoint
 // field; hard-source cells hold prescribed values, so their adjoint is zero.
-template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE>
-__global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
+template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE, bool SR>
+#ifndef VIRS_DENSE_WAVES
+#define VIRS_DENSE_WAVES 1 // minimum waves per SIMD the register allocation must allow
+#endif
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DENSE_WAVES))) void k_dense(const DenseParams p)
 {
     using S = Store<T>;
     using C = typename S::C;
@@ -521,6 +568,8 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
 
         const long long v = (long long)pchunk * kBlock + threadIdx.x;
         const bool active = v < p.nvec_plane;
+        // lanes past the plane's last vector read cell 0 of the plane, a
+        // border cell whose code marks it zero, and store nothing
         const long long cell0 = active ? v * V : 0;
         const int y = (int)(cell0 / nx);
         const int x0 = (int)(cell0 - (long long)y * nx);
@@ -530,16 +579,25 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
         const long long dys = (active && y > 0) ? nx : 0;
         const bool need_l = active && lane == 0 && x0 > 0;
         const bool need_r = active && lane == 63 && x0 + V < nx;
+        // per-lane byte offsets inside a plane (a plane is far below 4 GiB);
+        // the plane's base address is uniform across the block
+        const uint32_t oc = (uint32_t)cell0 * (uint32_t)sizeof(T);
+        const uint32_t on = oc + (uint32_t)dyn * (uint32_t)sizeof(T);
+        const uint32_t os = oc - (uint32_t)dys * (uint32_t)sizeof(T);
+        const uint32_t ob = (uint32_t)cell0;
 
         auto issue = [&](int z, Stage &st) {
-            const long long base = (long long)z * sxy + cell0;
-            st.pp = vload<VT, false>(reinterpret_cast<const VT *>(P + base + (z + 1 < nz ? sxy : 0)));
-            st.q = vload<VT, true>(reinterpret_cast<const VT *>(Q + base));
-            st.cd = vload<VB, true>(reinterpret_cast<const VB *>(p.code + base));
-            st.yn = vload<VT, false>(reinterpret_cast<const VT *>(P + base + dyn));
-            st.ys = vload<VT, false>(reinterpret_cast<const VT *>(P + base - dys));
-            if (need_l) st.tl = P[base - 1];
-            if (need_r) st.tr = P[base + V];
+            const char *Pz = reinterpret_cast<const char *>(P + (long long)z * sxy);
+            const char *Pzp = reinterpret_cast<const char *>(P + (long long)z * sxy + (z + 1 < nz ? sxy : 0));
+            const char *Qz = reinterpret_cast<const char *>(Q + (long long)z * sxy);
+            const uint8_t *Cz = p.code + (long long)z * sxy;
+            st.pp = vload<VT, false>(reinterpret_cast<const VT *>(Pzp + oc));
+            st.q = vload<VT, true>(reinterpret_cast<const VT *>(Qz + oc));
+            st.cd = vload<VB, true>(reinterpret_cast<const VB *>(Cz + ob));
+            st.yn = vload<VT, false>(reinterpret_cast<const VT *>(Pz + on));
+            st.ys = vload<VT, false>(reinterpret_cast<const VT *>(Pz + os));
+            if (need_l) st.tl = reinterpret_cast<const T *>(Pz + oc)[-1];
+            if (need_r) st.tr = reinterpret_cast<const T *>(Pz + oc)[V];
         };
 
         C B0[V], B1[V], B2[V];
@@ -559,17 +617,18 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
         // nxt receives the loads of plane z+1 before plane z is computed.
         auto body = [&](const int z, C(&PM)[V], C(&PC)[V], C(&PP)[V], Stage &cur, Stage &nxt) {
             const long long base = (long long)z * sxy + cell0;
-            const bool zp_ok = z + 1 < nz;
             C qv[V], yn[V], ys[V];
             uint32_t cd[V];
 #pragma unroll
             for (int j = 0; j < V; ++j) {
-                PP[j] = zp_ok ? S::ld(cur.pp.v[j]) : (C)0;
+                // at z = nz-1 this is plane z itself; that plane is border,
+                // so the value is never used
+                PP[j] = S::ld(cur.pp.v[j]);
                 if constexpr (S::kScaled) qv[j] = S::ld(cur.q.v[j]) * kq;
                 else qv[j] = S::ld(cur.q.v[j]);
                 yn[j] = S::ld(cur.yn.v[j]);
                 ys[j] = S::ld(cur.ys.v[j]);
-                cd[j] = active ? (uint32_t)cur.cd.v[j] : (uint32_t)(kTypeZero << 6);
+                cd[j] = (uint32_t)cur.cd.v[j];
             }
             C left = __shfl_up(PC[V - 1], 1);
             C right = __shfl_down(PC[0], 1);
@@ -582,6 +641,26 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
             for (int j = 0; j < V; ++j) {
                 const uint32_t c = cd[j];
                 const C pcv = PC[j];
+                if constexpr (S::kScaled) {
+                    C sum = (j + 1 < V ? PC[(j + 1) % V] : right) + (j > 0 ? PC[(j + V - 1) % V] : left);
+                    sum = sum + yn[j];
+                    sum = sum + ys[j];
+                    sum = sum + PP[j];
+                    sum = sum + PM[j];
+                    const C lap = (sum - (C)c * pcv) * ih2; // (C)c = kw for the cells that update
+                    C a = (C)2, b = (C)1, g = (C)1;
+                    if constexpr (SPONGE) {
+                        const int d = edge_dist(x0 + j, y, z, nx, ny, nz);
+                        if (d < p.sp.w) {
+                            a = (C)sp_tab[0][d];
+                            b = (C)sp_tab[1][d];
+                            g = (C)sp_tab[2][d];
+                        }
+                    }
+                    const C upd = (a * pcv - b * qv[j]) + g * (c2dt2 * lap);
+                    o[j] = bsel(bit_mask(c, 6), (C)0, bsel(bit_mask(c, 7), hard_u, upd));
+                    continue;
+                }
                 const C xp = bsel(bit_mask(c, 0), pcv, j + 1 < V ? PC[(j + 1) % V] : right);
                 const C xm = bsel(bit_mask(c, 1), pcv, j > 0 ? PC[(j + V - 1) % V] : left);
                 const C yp = bsel(bit_mask(c, 2), pcv, yn[j]);
@@ -614,7 +693,7 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
             if constexpr (S::kScaled) {
 #pragma unroll
                 for (int j = 0; j < V; ++j) lmax = fmaxf(lmax, fabsf((float)o[j]));
-                if (p.sr) {
+                if constexpr (SR) {
                     const uint32_t hk = hash_hk((unsigned long long)(base & ~1ll), n, kSaltDense);
                     const uint32_t lo = (uint32_t)(base & ~1ll) * kHashK1;
                     if constexpr (V == 1) {
@@ -639,7 +718,7 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
 #pragma unroll
                 for (int j = 0; j < V; ++j) out.v[j] = S::st(o[j]);
             }
-            if (active) vstore<VT, true>(reinterpret_cast<VT *>(R + base), out);
+            if (active) vstore<VT, true>(reinterpret_cast<VT *>(reinterpret_cast<char *>(R + (long long)z * sxy) + oc), out);
         };
 
         for (int z = z0; z < z1; z += 6) {
@@ -739,7 +818,7 @@ __global__ __launch_bounds__(kBlock) void k_dense(const DenseParams p)
                     r = boundary_update_pre<C>(p.bs, ev[u], n, pp - qq, r, x1[u], x2[u], y1[u], y2[u]);
                     if constexpr (S::kScaled) {
                         lmax = fmaxf(lmax, fabsf((float)(r / kphys)));
-                        R[bv[u]] = store_round<T>(r * (kout / kphys), p.sr, (unsigned long long)bv[u], n,
+                        R[bv[u]] = store_round<T>(r * (kout / kphys), SR, (unsigned long long)bv[u], n,
                                                   kSaltBoundary);
                     } else {
                         R[bv[u]] = S::st(r);
@@ -769,6 +848,11 @@ struct TailParams {
     void *R;
     const uint32_t *bidx;    // compact list; bs.nb = 0 when the stencil fused it
     BoundaryState bs;
+    // Stored pressures of the list cells as a two-slot ring by step parity:
+    // before step n, hv[n&1] holds p^n and hv[(n+1)&1] p^{n-1} at each
+    // entry's cell, in the field's storage type; step n replaces p^{n-1} by
+    // the p^{n+1} it stores.  Null: read p^n and p^{n-1} from the fields.
+    void *hv[2];
     StepCtl *ctl;
     int slot, next_slot;     // reads nring[slot], writes nring[next_slot]
     int nsoft;
@@ -805,11 +889,22 @@ __device__ inline void sponge_g(const TailParams &p, long long idx, C &g)
 // list when it was not fused into the stencil, then soft sources (added to
 // the corrected field), receiver gather, and the step counter.  Point cells
 // on the list are handled by their entry's thread, the others by block 0.
+// A thread handles tail_batch<T>() entries (block b covers entries
+// [b*U*kBlock, (b+1)*U*kBlock), lane-contiguous per batch slot) and issues
+// all their loads before any store: the entries are distinct cells, so no
+// store of the batch feeds a load of it.  p^n and p^{n-1} come from the
+// coalesced ring hv instead of two scattered gathers.
+template <typename T>
+constexpr int tail_batch()
+{
+    return Store<T>::kScaled ? 2 : 1; // measured: fewer 16-bit blocks (and |p| atomics) pay
+}
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
 {
     using S = Store<T>;
     using C = typename S::C;
+    constexpr int U = tail_batch<T>();
     const long long n = p.ctl->nring[p.slot];
     C sP = (C)1, sQ = (C)1, sR = (C)1, sRi = (C)1;
     if constexpr (S::kScaled) {
@@ -826,14 +921,39 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
     const bool in_ir = n >= 0 && n < p.ir_cap;
     float lmax = 0.f;
 
-    const long long e = (long long)blockIdx.x * kBlock + threadIdx.x;
-    if (e < p.bs.nb) {
-        const long long b = p.bidx[e];
-        C pv = S::ld(P[b]), qv = S::ld(Q[b]), r = S::ld(R[b]);
-        if constexpr (S::kScaled) pv = pv * sP, qv = qv * sQ, r = r * sRi;
-        r = boundary_update<C>(p.bs, e, n, pv - qv, r);
+    long long ev[U], bv[U];
+    bool ok[U];
+    C pv[U], qv[U], rv[U];
+    double x1[U], x2[U], y1[U], y2[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        ev[u] = ((long long)blockIdx.x * U + u) * kBlock + threadIdx.x;
+        ok[u] = ev[u] < p.bs.nb;
+        bv[u] = ok[u] ? (long long)p.bidx[ev[u]] : 0;
+    }
+    const T *hvn = (const T *)p.hv[n & 1], *hvq = (const T *)p.hv[(n + 1) & 1];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!ok[u]) continue;
+        if (hvn) {
+            pv[u] = S::ld(hvn[ev[u]]);
+            qv[u] = S::ld(hvq[ev[u]]);
+        } else {
+            pv[u] = S::ld(P[bv[u]]);
+            qv[u] = S::ld(Q[bv[u]]);
+        }
+        rv[u] = S::ld(R[bv[u]]);
+        load_taps<C>(p.bs, ev[u], n, x1[u], x2[u], y1[u], y2[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!ok[u]) continue;
+        const long long e = ev[u], b = bv[u];
+        C pp = pv[u], qq = qv[u], r = rv[u];
+        if constexpr (S::kScaled) pp = pp * sP, qq = qq * sQ, r = r * sRi;
+        r = boundary_update_pre<C>(p.bs, e, n, pp - qq, r, x1[u], x2[u], y1[u], y2[u]);
         T t = store_round<T>(r * sR, p.sr, (unsigned long long)b, n, kSaltBoundary);
-        if constexpr (S::kScaled) lmax = fabsf((float)r);
+        if constexpr (S::kScaled) lmax = fmaxf(lmax, fabsf((float)r));
         // a soft source on this cell is added to the stored corrected value
         for (int k = 0; k < p.nsoft; ++k)
             if (p.soft_bpos[k] == e && in_sig) {
@@ -844,11 +964,14 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
                 t = store_round<T>(r2 * sR, p.sr, (unsigned long long)b, n, kSaltSource);
             }
         R[b] = t;
-        C stored = S::ld(t);
-        if constexpr (S::kScaled) stored = stored * sRi;
+        if (hvn) ((T *)p.hv[(n + 1) & 1])[e] = t;
         if (in_ir)
             for (int k = 0; k < p.nrcv; ++k)
-                if (p.rcv_bpos[k] == e) p.ir[(long long)k * p.ir_cap + n] = (double)stored;
+                if (p.rcv_bpos[k] == e) {
+                    C stored = S::ld(t);
+                    if constexpr (S::kScaled) stored = stored * sRi;
+                    p.ir[(long long)k * p.ir_cap + n] = (double)stored;
+                }
     }
     if (blockIdx.x == 0) {
         const int k = threadIdx.x;
@@ -980,6 +1103,20 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
         for (int w = 1; w < kWaves; ++w) s = s + part[w][threadIdx.x];
         double *dst = p.grad_part + (long long)blockIdx.x * 5 * bs.nmat + threadIdx.x;
         *dst = *dst + s;
+    }
+}
+
+// Seeds the ring of list-cell pressures from the fields before step n:
+// pn = hv[n&1] receives p^n, pq = hv[(n+1)&1] receives p^{n-1}.
+template <typename T>
+__global__ void k_stash(const T *__restrict__ P, const T *__restrict__ Q, const uint32_t *__restrict__ bidx,
+                        T *__restrict__ pn, T *__restrict__ pq, long long nb)
+{
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < nb;
+         e += (long long)gridDim.x * blockDim.x) {
+        const uint32_t b = bidx[e];
+        pn[e] = P[b];
+        pq[e] = Q[b];
     }
 }
 

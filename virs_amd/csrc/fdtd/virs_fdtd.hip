@@ -46,6 +46,7 @@ struct virs_ctx {
     // filter state rings (BoundaryState::xs, ys): st[0..1] filter inputs in
     // the compute type (4 or 8 bytes), st[2..3] outputs in FP64
     void *st[4] = {nullptr, nullptr, nullptr, nullptr};
+    void *hv[2] = {nullptr, nullptr}; // list-cell pressures by step parity (TailParams::hv)
     std::vector<uint32_t> h_bidx;
     std::vector<uint8_t> h_bmat;
 
@@ -254,6 +255,8 @@ TailParams tail_params(const virs_ctx *c, const void *P, const void *Q, void *R,
     t.nz = (int)c->nz;
     t.sp = c->sp;
     t.sr = c->d.rounding == VIRS_ROUND_STOCHASTIC;
+    t.hv[0] = c->fused ? nullptr : c->hv[0];
+    t.hv[1] = c->fused ? nullptr : c->hv[1];
     return t;
 }
 
@@ -284,10 +287,10 @@ int dense_grid(const virs_ctx *c, K kernel, int &zranges)
     return (int)std::max<long long>(1, std::min<long long>(g, W));
 }
 
-template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE>
+template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE, bool SR>
 hipError_t launch_dense_k(const virs_ctx *c, const DenseParams &p)
 {
-    auto k = k_dense<T, V, ADJ, FUSED, SPONGE>;
+    auto k = k_dense<T, V, ADJ, FUSED, SPONGE, SR>;
     DenseParams q = p;
     const int grid = dense_grid(c, k, q.zranges);
     hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, c->stream, q);
@@ -301,13 +304,30 @@ hipError_t launch_dense_v(const virs_ctx *c, const DenseParams &p)
         return hipErrorInvalidValue;
     } else {
         const bool sponge = p.sp.w > 0;
-        if constexpr (!ADJ) {
-            if (p.bstart) {
-                return sponge ? launch_dense_k<T, V, false, true, true>(c, p)
-                              : launch_dense_k<T, V, false, true, false>(c, p);
+        if constexpr (Store<T>::kScaled) {
+            // stochastic or nearest storage rounding, chosen at compile time
+            if (p.sr) {
+                if (p.bstart)
+                    return sponge ? launch_dense_k<T, V, false, true, true, true>(c, p)
+                                  : launch_dense_k<T, V, false, true, false, true>(c, p);
+                return sponge ? launch_dense_k<T, V, false, false, true, true>(c, p)
+                              : launch_dense_k<T, V, false, false, false, true>(c, p);
             }
+            if (p.bstart)
+                return sponge ? launch_dense_k<T, V, false, true, true, false>(c, p)
+                              : launch_dense_k<T, V, false, true, false, false>(c, p);
+            return sponge ? launch_dense_k<T, V, false, false, true, false>(c, p)
+                          : launch_dense_k<T, V, false, false, false, false>(c, p);
+        } else {
+            if constexpr (!ADJ) {
+                if (p.bstart) {
+                    return sponge ? launch_dense_k<T, V, false, true, true, false>(c, p)
+                                  : launch_dense_k<T, V, false, true, false, false>(c, p);
+                }
+            }
+            return sponge ? launch_dense_k<T, V, ADJ, false, true, false>(c, p)
+                          : launch_dense_k<T, V, ADJ, false, false, false>(c, p);
         }
-        return sponge ? launch_dense_k<T, V, ADJ, false, true>(c, p) : launch_dense_k<T, V, ADJ, false, false>(c, p);
     }
 }
 
@@ -332,16 +352,52 @@ hipError_t launch_dense(const virs_ctx *c, const DenseParams &p, bool adj)
     }
 }
 
+template <typename T>
+void launch_tail_t(const virs_ctx *c, const TailParams &t)
+{
+    const long long per = (long long)kBlock * tail_batch<T>();
+    const int g = (int)std::max<long long>(1, (t.bs.nb + per - 1) / per);
+    hipLaunchKernelGGL(k_tail<T>, dim3(g), dim3(kBlock), 0, c->stream, t);
+}
+
 hipError_t launch_tail(const virs_ctx *c, const TailParams &t)
 {
-    const int g = tail_grid(t.bs.nb);
     switch (c->d.precision) {
-    case VIRS_FP16: hipLaunchKernelGGL(k_tail<f16>, dim3(g), dim3(kBlock), 0, c->stream, t); break;
-    case VIRS_BF16: hipLaunchKernelGGL(k_tail<bf16>, dim3(g), dim3(kBlock), 0, c->stream, t); break;
-    case VIRS_FP64: hipLaunchKernelGGL(k_tail<double>, dim3(g), dim3(kBlock), 0, c->stream, t); break;
-    default: hipLaunchKernelGGL(k_tail<float>, dim3(g), dim3(kBlock), 0, c->stream, t); break;
+    case VIRS_FP16: launch_tail_t<f16>(c, t); break;
+    case VIRS_BF16: launch_tail_t<bf16>(c, t); break;
+    case VIRS_FP64: launch_tail_t<double>(c, t); break;
+    default: launch_tail_t<float>(c, t); break;
     }
     return hipGetLastError();
+}
+
+// Seeds TailParams::hv from the current fields for the next step n.
+int stash_boundary(virs_ctx *c, long long n)
+{
+    if (!c->nb || !c->hv[0]) return VIRS_OK;
+    const void *P = c->f[c->ip], *Q = c->f[c->iq];
+    void *pn = c->hv[n & 1], *pq = c->hv[(n + 1) & 1];
+    const int g = grid_for(c->nb);
+    switch (c->d.precision) {
+    case VIRS_FP16:
+        hipLaunchKernelGGL(k_stash<f16>, dim3(g), dim3(256), 0, c->stream, (const f16 *)P, (const f16 *)Q, c->bidx,
+                           (f16 *)pn, (f16 *)pq, c->nb);
+        break;
+    case VIRS_BF16:
+        hipLaunchKernelGGL(k_stash<bf16>, dim3(g), dim3(256), 0, c->stream, (const bf16 *)P, (const bf16 *)Q,
+                           c->bidx, (bf16 *)pn, (bf16 *)pq, c->nb);
+        break;
+    case VIRS_FP64:
+        hipLaunchKernelGGL(k_stash<double>, dim3(g), dim3(256), 0, c->stream, (const double *)P,
+                           (const double *)Q, c->bidx, (double *)pn, (double *)pq, c->nb);
+        break;
+    default:
+        hipLaunchKernelGGL(k_stash<float>, dim3(g), dim3(256), 0, c->stream, (const float *)P, (const float *)Q,
+                           c->bidx, (float *)pn, (float *)pq, c->nb);
+        break;
+    }
+    HIPC(c, hipGetLastError());
+    return VIRS_OK;
 }
 
 hipError_t launch_tail_adj(const virs_ctx *c, const TailParams &t)
@@ -456,6 +512,8 @@ int reset_run(virs_ctx *c)
     c->ip = 0;
     c->iq = 1;
     c->ir = 2;
+    rc = stash_boundary(c, 0);
+    if (rc) return rc;
     c->steps = c->counted = 0;
     c->stopped = false;
     c->rms_state = c->percent = 0.f;
@@ -573,6 +631,7 @@ int virs_destroy(virs_ctx *c)
     for (auto &p : c->f) dfree(p);
     for (auto &p : c->af) dfree(p);
     for (auto &p : c->st) dfree(p);
+    for (auto &p : c->hv) dfree(p);
     for (auto &p : c->ast) dfree(p);
     for (auto &p : c->cp_field) dfree(p);
     for (auto &p : c->cp_state) dfree(p);
@@ -628,9 +687,22 @@ int virs_upload_flags(virs_ctx *c, const uint8_t *flags)
     int rc = dalloc(c, &dflags, (size_t)c->N);
     if (rc) return rc;
     hipError_t e = hipMemcpyAsync(dflags, flags, (size_t)c->N, hipMemcpyHostToDevice, c->stream);
+    const bool s16 = c->d.precision == VIRS_FP16 || c->d.precision == VIRS_BF16;
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_build_code, dim3(4096), dim3(256), 0, c->stream, dflags, c->code,
-                           (int)c->nx, (int)c->ny, (int)c->nz);
+        if (s16)
+            hipLaunchKernelGGL(k_build_code16, dim3(4096), dim3(256), 0, c->stream, dflags, c->code,
+                               (int)c->nx, (int)c->ny, (int)c->nz);
+        else
+            hipLaunchKernelGGL(k_build_code, dim3(4096), dim3(256), 0, c->stream, dflags, c->code,
+                               (int)c->nx, (int)c->ny, (int)c->nz);
+        e = hipGetLastError();
+    }
+    // 16-bit storage keeps walls and border cells at zero in every field
+    for (int k = 0; k < 3 && s16 && e == hipSuccess; ++k) {
+        if (c->d.precision == VIRS_FP16)
+            hipLaunchKernelGGL(k_zero_walls<f16>, dim3(4096), dim3(256), 0, c->stream, (f16 *)c->f[k], c->code, c->N);
+        else
+            hipLaunchKernelGGL(k_zero_walls<bf16>, dim3(4096), dim3(256), 0, c->stream, (bf16 *)c->f[k], c->code, c->N);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -674,18 +746,23 @@ int virs_upload_boundary(virs_ctx *c, const uint32_t *idx, const uint8_t *mat, i
     dfree(c->bidx);
     dfree(c->bmat);
     for (auto &p : c->st) dfree(p);
+    for (auto &p : c->hv) dfree(p);
     c->nb = nb;
     c->h_bidx.assign(idx, idx + nb);
     if (mat) c->h_bmat.assign(mat, mat + nb);
     else c->h_bmat.clear();
     int rc = dalloc(c, &c->bidx, sizeof(uint32_t) * nb);
     for (int k = 0; k < 4 && !rc; ++k) rc = dalloc(c, &c->st[k], state_bytes(c, k));
+    for (int k = 0; k < 2 && !rc; ++k) rc = dalloc(c, &c->hv[k], c->esz * (size_t)std::max<long long>(1, nb));
     if (!rc && mat) rc = dalloc(c, &c->bmat, (size_t)nb);
     if (rc) return rc;
     if (nb) HIPC(c, hipMemcpy(c->bidx, idx, sizeof(uint32_t) * nb, hipMemcpyHostToDevice));
     if (nb && mat) HIPC(c, hipMemcpy(c->bmat, mat, (size_t)nb, hipMemcpyHostToDevice));
     for (int k = 0; k < 4; ++k)
         if (nb) HIPC(c, hipMemset(c->st[k], 0, state_bytes(c, k)));
+    rc = stash_boundary(c, c->steps);
+    if (rc) return rc;
+    HIPC(c, hipStreamSynchronize(c->stream));
     drop_graphs(c);
     rc = refresh_fusion(c);
     if (rc) return rc;
@@ -1017,8 +1094,16 @@ int virs_write_field_f64(virs_ctx *c, int32_t which, const double *in)
     const int g = 4096;
     if (err == hipSuccess) {
         switch (c->d.precision) {
-        case VIRS_FP16: hipLaunchKernelGGL(k_from_f64<f16>, dim3(g), dim3(256), 0, c->stream, tmp, (f16 *)dst, c->N, e); break;
-        case VIRS_BF16: hipLaunchKernelGGL(k_from_f64<bf16>, dim3(g), dim3(256), 0, c->stream, tmp, (bf16 *)dst, c->N, e); break;
+        case VIRS_FP16:
+            hipLaunchKernelGGL(k_from_f64<f16>, dim3(g), dim3(256), 0, c->stream, tmp, (f16 *)dst, c->N, e);
+            if (c->have_flags)
+                hipLaunchKernelGGL(k_zero_walls<f16>, dim3(g), dim3(256), 0, c->stream, (f16 *)dst, c->code, c->N);
+            break;
+        case VIRS_BF16:
+            hipLaunchKernelGGL(k_from_f64<bf16>, dim3(g), dim3(256), 0, c->stream, tmp, (bf16 *)dst, c->N, e);
+            if (c->have_flags)
+                hipLaunchKernelGGL(k_zero_walls<bf16>, dim3(g), dim3(256), 0, c->stream, (bf16 *)dst, c->code, c->N);
+            break;
         case VIRS_FP64: hipLaunchKernelGGL(k_from_f64<double>, dim3(g), dim3(256), 0, c->stream, tmp, (double *)dst, c->N, e); break;
         default: hipLaunchKernelGGL(k_from_f64<float>, dim3(g), dim3(256), 0, c->stream, tmp, (float *)dst, c->N, e); break;
         }
@@ -1027,6 +1112,9 @@ int virs_write_field_f64(virs_ctx *c, int32_t which, const double *in)
     if (err == hipSuccess) err = hipStreamSynchronize(c->stream);
     (void)hipFree(tmp);
     if (err != hipSuccess) return fail(c, VIRS_ERR_HIP, "write field: %s", hipGetErrorString(err));
+    rc = stash_boundary(c, c->steps);
+    if (rc) return rc;
+    HIPC(c, hipStreamSynchronize(c->stream));
     return VIRS_OK;
 }
 
@@ -1035,7 +1123,7 @@ int virs_memory_bytes(const virs_ctx *c, int64_t *bytes)
     if (!c || !bytes) return VIRS_ERR_ARG;
     long long b = 3 * (long long)c->esz * c->N + c->N; // fields + cell codes
     if (c->normals) b += c->N;
-    b += c->nb * (4 + 2 * (long long)compute_bytes(c) + 16 + (c->bmat ? 1 : 0));
+    b += c->nb * (4 + 2 * (long long)compute_bytes(c) + 16 + 2 * (long long)c->esz + (c->bmat ? 1 : 0));
     b += (long long)sizeof(double) * std::max(1, c->nrcv) * std::max<long long>(1, c->ir_cap);
     b += (long long)c->cp_field.size() * (long long)c->esz * c->N;
     b += (long long)c->cp_state.size() * (2 * (long long)compute_bytes(c) + 16) * c->nb;
@@ -1051,7 +1139,12 @@ int virs_time_steps(virs_ctx *c, int64_t nsteps, double *total_ms, double *dense
     int rc = check_ready(c);
     if (rc) return rc;
     if (nsteps <= 0) return VIRS_ERR_ARG;
-    std::vector<hipEvent_t> ev((size_t)nsteps * 3);
+    // Launch durations are taken on about 64 steps spread over the run (events
+    // around each of their launches); the other steps are launched plainly, so
+    // the event records barely touch the loop time.
+    const long long stride = std::max<long long>(1, nsteps / 64);
+    const long long nsamp = (nsteps + stride - 1) / stride;
+    std::vector<hipEvent_t> ev((size_t)nsamp * 3);
     for (auto &e : ev) HIPC(c, hipEventCreate(&e));
     hipEvent_t t0, t1;
     HIPC(c, hipEventCreate(&t0));
@@ -1059,17 +1152,17 @@ int virs_time_steps(virs_ctx *c, int64_t nsteps, double *total_ms, double *dense
     HIPC(c, hipStreamSynchronize(c->stream));
     HIPC(c, hipEventRecord(t0, c->stream));
     for (long long s = 0; s < nsteps && !rc; ++s) {
-        rc = launch_step(c, nullptr, nullptr, 0, true, &ev[(size_t)s * 3]);
+        rc = launch_step(c, nullptr, nullptr, 0, true, s % stride == 0 ? &ev[(size_t)(s / stride) * 3] : nullptr);
         ++c->steps;
     }
     c->counted = c->steps;
     HIPC(c, hipEventRecord(t1, c->stream));
     HIPC(c, hipEventSynchronize(t1));
     double dsum = 0, bsum = 0;
-    for (long long s = 0; s < nsteps; ++s) {
+    for (long long k = 0; k < nsamp; ++k) {
         float a = 0, b = 0;
-        HIPC(c, hipEventElapsedTime(&a, ev[s * 3], ev[s * 3 + 1]));
-        HIPC(c, hipEventElapsedTime(&b, ev[s * 3 + 1], ev[s * 3 + 2]));
+        HIPC(c, hipEventElapsedTime(&a, ev[k * 3], ev[k * 3 + 1]));
+        HIPC(c, hipEventElapsedTime(&b, ev[k * 3 + 1], ev[k * 3 + 2]));
         dsum += a;
         bsum += b;
     }
@@ -1080,8 +1173,8 @@ int virs_time_steps(virs_ctx *c, int64_t nsteps, double *total_ms, double *dense
     (void)hipEventDestroy(t1);
     if (rc) return rc;
     if (total_ms) *total_ms = tot;
-    if (dense_ms) *dense_ms = dsum / (double)nsteps;
-    if (boundary_ms) *boundary_ms = bsum / (double)nsteps;
+    if (dense_ms) *dense_ms = dsum / (double)nsamp;
+    if (boundary_ms) *boundary_ms = bsum / (double)nsamp;
     return VIRS_OK;
 }
 
@@ -1177,7 +1270,8 @@ static int recompute_segment(virs_ctx *c, int k)
     }
     hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl, 0, s0, 0);
     HIPC(c, hipGetLastError());
-    int rc = seed_trace(c, buf, s0);
+    int rc = stash_boundary(c, s0);
+    if (!rc) rc = seed_trace(c, buf, s0);
     for (long long s = s0; s < s1 && !rc; ++s) rc = launch_step(c, c->trace_d[buf], c->trace_y[buf], s0, false);
     return rc;
 }
