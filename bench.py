@@ -8,8 +8,10 @@ and 8 receivers, with fields already resident in HBM.  Prints one JSON line.
 
 roofline: the stencil launch's algorithmic bytes N*(3s+1) (read p^n and
 p^{n-1}, write p^{n+1}, read one code byte per cell) divided by its mean
-launch time, taken with HIP events around the launches of about 64 steps
-spread over the timed region, on the library's own stream.
+launch time, taken with HIP events around the two launches of every 31st
+step of the timed region (the steps between are replayed as 30-step HIP
+graphs; --no-graph launches every step and samples about 64), on the
+library's own stream.
 cpu_baseline: the CPU oracle (the same scheme, OpenMP over z) on a 128^3
 corner crop of the same room, on this host's cores.
 """
@@ -56,10 +58,11 @@ def step_bytes(n, nb, nrcv, s):
     return n * (3 * s + 1) + nb * (69 + 4 * s) + nrcv * s
 
 
-def run_gpu(prep, precision, steps, warmup, dense_blocks=0, fuse=False, vw=0):
+def run_gpu(prep, precision, steps, warmup, dense_blocks=0, fuse=False, vw=0, graph=True):
     from virs_amd.fdtd import BYTES, setup
     ts = time.perf_counter()
-    f = setup(prep, precision, dense_blocks=dense_blocks, fuse_boundary=fuse, vector_width=vw)
+    f = setup(prep, precision, use_graph=graph, dense_blocks=dense_blocks, fuse_boundary=fuse,
+              vector_width=vw)
     f.synchronize()
     setup_s = time.perf_counter() - ts
     f.start(warmup + steps)
@@ -230,6 +233,7 @@ def main():
     ap.add_argument("--fuse", action="store_true", help="run the boundary filter inside the stencil pass")
     ap.add_argument("--vw16", type=int, default=0, help="vector width of the 16-bit variants (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch every step individually")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
     ap.add_argument("--large", action="store_true",
                     help="only the 1536x1024x768 FP16 room and the FP32 adjoint on its 512^3 crop")
@@ -267,7 +271,8 @@ def main():
     nb = len(prep.room.bidx)
     nrcv = len(prep.rcv_idx)
 
-    r = run_gpu(prep, args.precision, args.steps, args.warmup, args.dense_blocks, args.fuse)
+    r = run_gpu(prep, args.precision, args.steps, args.warmup, args.dense_blocks, args.fuse,
+                graph=not args.no_graph)
     ms = r["total_ms"] / args.steps
     value = n * args.steps / (r["total_ms"] / 1e3) / 1e9
     s = r["s"]
@@ -289,7 +294,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "random room 512x384x256 (6 boxes + 4 tilted slabs, 3 biquads, "
                                "8 receivers, Gaussian-derivative source)",
-                   "cells": n, "boundary_cells": nb, "storage": args.precision},
+                   "cells": n, "boundary_cells": nb, "storage": args.precision,
+                   "launch": "individual launches" if args.no_graph else "30-step HIP graphs"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(args.precision) if not args.fuse and not args.dense_blocks else None,
@@ -309,12 +315,12 @@ def main():
     variants = {}
     for prec in [v for v in args.variants.split(",") if v and v != args.precision]:
         rv = run_gpu(prep, prec, args.steps, args.warmup, args.dense_blocks, args.fuse,
-                     args.vw16 if prec in ("fp16", "bf16") else 0)
+                     args.vw16 if prec in ("fp16", "bf16") else 0, graph=not args.no_graph)
         vv = n * args.steps / (rv["total_ms"] / 1e3) / 1e9
         db = n * (3 * rv["s"] + 1)
         variants[prec] = {"value": round(vv, 3), "speedup_vs_main": round(vv / value, 3),
                           "dense_gbs": round(db / (rv["dense_ms"] / 1e3) / 1e9, 1),
-                          "dense_ms": round(rv["dense_ms"], 5)}
+                          "dense_ms": round(rv["dense_ms"], 5), "boundary_kernel_ms": round(rv["bnd_ms"], 5)}
     if variants:
         out["variants"] = variants
     if not args.no_adjoint:

@@ -384,7 +384,8 @@ int oracle_adjoint_f64(const oracle_desc *d, int64_t nsteps, const double *dL_di
  *   - the exponent of step n's field is 14 - k (clamped to [-60, 60]),
  *     est = m*2^k, where
  *     est = 16*max(M(n-1), M(n-2)) + |hard(n)| + sum_k |soft_k(n)| and M(s)
- *     is the largest |value| computed for step s before storage rounding;
+ *     is the largest |value| the stencil computed for step s before storage
+ *     rounding (not the boundary-corrected or source-added values);
  *   - storage rounding is nearest-even or stochastic: 16 bits of a
  *     counter-based hash of (cell, step, salt) are added below the 16-bit
  *     format's last place before truncation (mix32 below);
@@ -610,7 +611,6 @@ int oracle_run_s16(const oracle_desc *d, int bf16, int sr, int64_t nsteps, doubl
             s4[1] = s4[0]; s4[0] = in; s4[3] = s4[2]; s4[2] = yv;
             float r = ld16(&S, R[bi]) * sRi;
             r = (float)((double)r - (double)d->kconst * (yv * 0.1 + in * 0.9) * (double)0.4f);
-            if (fabsf(r) > mx) mx = fabsf(r);
             R[bi] = st16(&S, r * sR, (uint64_t)bi, n, SALT_BOUNDARY);
         }
         /* soft sources, on boundary cells too, add to the stored value */
@@ -619,7 +619,6 @@ int oracle_run_s16(const oracle_desc *d, int bf16, int sr, int64_t nsteps, doubl
             const int64_t si = d->soft_idx[k];
             float r = ld16(&S, R[si]) * sRi;
             r = r + (float)d->soft_sig[(int64_t)k * d->soft_len + n];
-            if (fabsf(r) > mx) mx = fabsf(r);
             R[si] = st16(&S, r * sR, (uint64_t)si, n, SALT_SOURCE);
         }
         ering[n & 3] = eR;

@@ -29,8 +29,11 @@ constexpr int kMaxPoints = 64;
 // phase's entry, which no block of the current launch reads, so the counter
 // advances without atomics and a captured graph of whole rotation cycles
 // replays correctly.  exp/maxabs are rings indexed by step & 3 holding each
-// stored field's power-of-two scale exponent and max |value| (physical
-// units, float bits: non-negative floats order like their bit patterns).
+// stored field's power-of-two scale exponent and the largest |value| the
+// stencil launch computed for it (physical units, float bits: non-negative
+// floats order like their bit patterns).  The boundary correction, a bounded
+// damping term, and the point sources, which enter the exponent's estimate
+// themselves, do not contribute to it.
 struct StepCtl {
     long long nring[3];
     int err;
@@ -817,7 +820,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                     }
                     r = boundary_update_pre<C>(p.bs, ev[u], n, pp - qq, r, x1[u], x2[u], y1[u], y2[u]);
                     if constexpr (S::kScaled) {
-                        lmax = fmaxf(lmax, fabsf((float)(r / kphys)));
                         R[bv[u]] = store_round<T>(r * (kout / kphys), SR, (unsigned long long)bv[u], n,
                                                   kSaltBoundary);
                     } else {
@@ -919,7 +921,6 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
     T *__restrict__ R = (T *)p.R;
     const bool in_sig = n >= 0 && n < p.soft_len;
     const bool in_ir = n >= 0 && n < p.ir_cap;
-    float lmax = 0.f;
 
     long long ev[U], bv[U];
     bool ok[U];
@@ -953,14 +954,12 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
         if constexpr (S::kScaled) pp = pp * sP, qq = qq * sQ, r = r * sRi;
         r = boundary_update_pre<C>(p.bs, e, n, pp - qq, r, x1[u], x2[u], y1[u], y2[u]);
         T t = store_round<T>(r * sR, p.sr, (unsigned long long)b, n, kSaltBoundary);
-        if constexpr (S::kScaled) lmax = fmaxf(lmax, fabsf((float)r));
         // a soft source on this cell is added to the stored corrected value
         for (int k = 0; k < p.nsoft; ++k)
             if (p.soft_bpos[k] == e && in_sig) {
                 C r2 = S::ld(t);
                 if constexpr (S::kScaled) r2 = r2 * sRi;
                 r2 = r2 + (C)p.soft_sig[(long long)k * p.soft_len + n];
-                if constexpr (S::kScaled) lmax = fmaxf(lmax, fabsf((float)r2));
                 t = store_round<T>(r2 * sR, p.sr, (unsigned long long)b, n, kSaltSource);
             }
         R[b] = t;
@@ -980,7 +979,6 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
             C r = S::ld(R[i]);
             if constexpr (S::kScaled) r = r * sRi;
             r = r + (C)p.soft_sig[(long long)k * p.soft_len + n];
-            if constexpr (S::kScaled) lmax = fmaxf(lmax, fabsf((float)r));
             R[i] = store_round<T>(r * sR, p.sr, (unsigned long long)i, n, kSaltSource);
         }
         __syncthreads();
@@ -990,17 +988,6 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
             p.ir[(long long)k * p.ir_cap + n] = (double)r;
         }
         if (k == 0) p.ctl->nring[p.next_slot] = n + 1;
-    }
-    if constexpr (S::kScaled) {
-        __shared__ float red[kWaves];
-        lmax = wave_max(lmax);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = lmax;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            float mx = red[0];
-            for (int w = 1; w < kWaves; ++w) mx = fmaxf(mx, red[w]);
-            atomicMax(&p.ctl->maxabs[n & 3][blockIdx.x % kSlots], __float_as_uint(mx));
-        }
     }
 }
 

@@ -1134,15 +1134,41 @@ int virs_memory_bytes(const virs_ctx *c, int64_t *bytes)
     return VIRS_OK;
 }
 
+// Builds the graphs of all three rotation phases that are missing.
+static int ensure_graphs(virs_ctx *c, int steps)
+{
+    const int sp = c->ip, sq = c->iq, sr = c->ir;
+    int rc = VIRS_OK;
+    for (int k = 0; k < 3 && !rc; ++k) {
+        if (!c->graph[c->ip] || c->graph_steps != steps) rc = build_graph(c, steps);
+        const int ip = c->ip; // the rotation of one step, as launch_step applies it
+        c->ip = c->ir;
+        c->ir = c->iq;
+        c->iq = ip;
+    }
+    c->ip = sp;
+    c->iq = sq;
+    c->ir = sr;
+    return rc;
+}
+
 int virs_time_steps(virs_ctx *c, int64_t nsteps, double *total_ms, double *dense_ms, double *boundary_ms)
 {
     int rc = check_ready(c);
     if (rc) return rc;
     if (nsteps <= 0) return VIRS_ERR_ARG;
-    // Launch durations are taken on about 64 steps spread over the run (events
-    // around each of their launches); the other steps are launched plainly, so
-    // the event records barely touch the loop time.
-    const long long stride = std::max<long long>(1, nsteps / 64);
+    // Launch durations are taken on a sample of the steps (events around its
+    // two launches).  With graphs, every sampled step is followed by one
+    // replay of a 30-step graph; otherwise about 64 steps spread over the run
+    // are sampled and the others launched plainly.  Either way the event
+    // records barely touch the loop time.
+    const int gsteps = 30;
+    const bool graphs = c->d.use_graph && nsteps > gsteps;
+    if (graphs) {
+        rc = ensure_graphs(c, gsteps);
+        if (rc) return rc;
+    }
+    const long long stride = graphs ? gsteps + 1 : std::max<long long>(1, nsteps / 64);
     const long long nsamp = (nsteps + stride - 1) / stride;
     std::vector<hipEvent_t> ev((size_t)nsamp * 3);
     for (auto &e : ev) HIPC(c, hipEventCreate(&e));
@@ -1151,30 +1177,47 @@ int virs_time_steps(virs_ctx *c, int64_t nsteps, double *total_ms, double *dense
     HIPC(c, hipEventCreate(&t1));
     HIPC(c, hipStreamSynchronize(c->stream));
     HIPC(c, hipEventRecord(t0, c->stream));
-    for (long long s = 0; s < nsteps && !rc; ++s) {
-        rc = launch_step(c, nullptr, nullptr, 0, true, s % stride == 0 ? &ev[(size_t)(s / stride) * 3] : nullptr);
+    long long s = 0, k = 0;
+    while (s < nsteps && !rc) {
+        rc = launch_step(c, nullptr, nullptr, 0, true, &ev[(size_t)k * 3]);
+        ++k;
+        ++s;
         ++c->steps;
+        long long plain = std::min<long long>(stride - 1, nsteps - s);
+        if (graphs)
+            while (!rc && plain >= gsteps) {
+                if (hipGraphLaunch(c->graph[c->ip], c->stream) != hipSuccess)
+                    rc = fail(c, VIRS_ERR_HIP, "graph launch failed");
+                plain -= gsteps;
+                s += gsteps;
+                c->steps += gsteps;
+            }
+        for (; plain > 0 && !rc; --plain) {
+            rc = launch_step(c);
+            ++s;
+            ++c->steps;
+        }
     }
     c->counted = c->steps;
-    HIPC(c, hipEventRecord(t1, c->stream));
-    HIPC(c, hipEventSynchronize(t1));
+    if (!rc) HIPC(c, hipEventRecord(t1, c->stream));
+    if (!rc) HIPC(c, hipEventSynchronize(t1));
     double dsum = 0, bsum = 0;
-    for (long long k = 0; k < nsamp; ++k) {
+    for (long long j = 0; j < k && !rc; ++j) {
         float a = 0, b = 0;
-        HIPC(c, hipEventElapsedTime(&a, ev[k * 3], ev[k * 3 + 1]));
-        HIPC(c, hipEventElapsedTime(&b, ev[k * 3 + 1], ev[k * 3 + 2]));
+        HIPC(c, hipEventElapsedTime(&a, ev[j * 3], ev[j * 3 + 1]));
+        HIPC(c, hipEventElapsedTime(&b, ev[j * 3 + 1], ev[j * 3 + 2]));
         dsum += a;
         bsum += b;
     }
     float tot = 0;
-    HIPC(c, hipEventElapsedTime(&tot, t0, t1));
+    if (!rc) HIPC(c, hipEventElapsedTime(&tot, t0, t1));
     for (auto &e : ev) (void)hipEventDestroy(e);
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     if (rc) return rc;
     if (total_ms) *total_ms = tot;
-    if (dense_ms) *dense_ms = dsum / (double)nsamp;
-    if (boundary_ms) *boundary_ms = bsum / (double)nsamp;
+    if (dense_ms) *dense_ms = dsum / (double)k;
+    if (boundary_ms) *boundary_ms = bsum / (double)k;
     return VIRS_OK;
 }
 
