@@ -527,6 +527,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
     const long long sxy = p.sxy;
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const C c2dt2 = (C)p.c2dt2, ih2 = (C)p.ih2;
+    const C c2ih2 = c2dt2 * ih2; // 16-bit scheme: c2dt2*ih2 folded into one factor
 
     __shared__ float sp_tab[SPONGE ? 3 : 1][SPONGE ? kMaxSponge : 1];
     if constexpr (SPONGE) {
@@ -627,11 +628,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                 // at z = nz-1 this is plane z itself; that plane is border,
                 // so the value is never used
                 PP[j] = S::ld(cur.pp.v[j]);
-                if constexpr (S::kScaled) qv[j] = S::ld(cur.q.v[j]) * kq;
-                else qv[j] = S::ld(cur.q.v[j]);
+                qv[j] = S::ld(cur.q.v[j]);
                 yn[j] = S::ld(cur.yn.v[j]);
                 ys[j] = S::ld(cur.ys.v[j]);
                 cd[j] = (uint32_t)cur.cd.v[j];
+            }
+            if constexpr (S::kScaled) {
+                // Q to P units; consecutive steps mostly share an exponent,
+                // and a multiply by one is exact, so it is skipped then
+                if (kq != (C)1)
+#pragma unroll
+                    for (int j = 0; j < V; ++j) qv[j] = qv[j] * kq;
             }
             C left = __shfl_up(PC[V - 1], 1);
             C right = __shfl_down(PC[0], 1);
@@ -650,7 +657,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                     sum = sum + ys[j];
                     sum = sum + PP[j];
                     sum = sum + PM[j];
-                    const C lap = (sum - (C)c * pcv) * ih2; // (C)c = kw for the cells that update
+                    const C lapc = (sum - (C)c * pcv) * c2ih2; // (C)c = kw for the cells that update
                     C a = (C)2, b = (C)1, g = (C)1;
                     if constexpr (SPONGE) {
                         const int d = edge_dist(x0 + j, y, z, nx, ny, nz);
@@ -660,7 +667,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                             g = (C)sp_tab[2][d];
                         }
                     }
-                    const C upd = (a * pcv - b * qv[j]) + g * (c2dt2 * lap);
+                    const C upd = (a * pcv - b * qv[j]) + g * lapc;
                     o[j] = bsel(bit_mask(c, 6), (C)0, bsel(bit_mask(c, 7), hard_u, upd));
                     continue;
                 }
@@ -696,26 +703,29 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
             if constexpr (S::kScaled) {
 #pragma unroll
                 for (int j = 0; j < V; ++j) lmax = fmaxf(lmax, fabsf((float)o[j]));
+                if (kout != (C)1) // P units to R's stored units (skipped when exact)
+#pragma unroll
+                    for (int j = 0; j < V; ++j) o[j] = o[j] * kout;
                 if constexpr (SR) {
                     const uint32_t hk = hash_hk((unsigned long long)(base & ~1ll), n, kSaltDense);
                     const uint32_t lo = (uint32_t)(base & ~1ll) * kHashK1;
                     if constexpr (V == 1) {
                         const uint32_t h = hash_pair(lo, hk);
-                        const float q0 = sr_bits<T>(o[0] * kout, (base & 1) ? (h >> 16) : (h & 0xffffu));
+                        const float q0 = sr_bits<T>(o[0], (base & 1) ? (h >> 16) : (h & 0xffffu));
                         T dummy;
                         cvt2_truncated<T>(q0, 0.f, out.v[0], dummy);
                     } else {
 #pragma unroll
                         for (int j = 0; j < V; j += 2) {
                             const uint32_t h = hash_pair(lo + (uint32_t)j * kHashK1, hk);
-                            const float qa = sr_bits<T>(o[j] * kout, h & 0xffffu);
-                            const float qb = sr_bits<T>(o[j + 1] * kout, h >> 16);
+                            const float qa = sr_bits<T>(o[j], h & 0xffffu);
+                            const float qb = sr_bits<T>(o[j + 1], h >> 16);
                             cvt2_truncated<T>(qa, qb, out.v[j], out.v[j + 1]);
                         }
                     }
                 } else {
 #pragma unroll
-                    for (int j = 0; j < V; ++j) out.v[j] = S::st(o[j] * kout);
+                    for (int j = 0; j < V; ++j) out.v[j] = S::st(o[j]);
                 }
             } else {
 #pragma unroll
@@ -724,6 +734,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
             if (active) vstore<VT, true>(reinterpret_cast<VT *>(reinterpret_cast<char *>(R + (long long)z * sxy) + oc), out);
         };
 
+        // planes rotate with period 3, load stages with period 2
         for (int z = z0; z < z1; z += 6) {
             body(z, B0, B1, B2, SA, SB);
             if (z + 1 >= z1) break;
