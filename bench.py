@@ -54,8 +54,11 @@ def host_cores():
 
 
 def step_bytes(n, nb, nrcv, s):
-    """Algorithmic bytes of one step: N(3s+1) + Nb(69+4s) + R s."""
-    return n * (3 * s + 1) + nb * (69 + 4 * s) + nrcv * s
+    """Algorithmic bytes of one step: N(3s+1) + Nb(41+5s) + R s.  Per list
+    entry: 4 index + 1 material + 8+4 filter-input ring (float) + 16+8
+    filter-output ring (FP64) + 2s+s pressure ring + s+s p^{n+1} read and
+    written."""
+    return n * (3 * s + 1) + nb * (41 + 5 * s) + nrcv * s
 
 
 def run_gpu(prep, precision, steps, warmup, dense_blocks=0, fuse=False, vw=0, graph=True):
@@ -108,6 +111,39 @@ def cpu_baseline(prep, crop=128, steps=200, warmup=5):
                   f"warm-up, FP32 oracle (same scheme, OpenMP over z)",
         "reference_cpu_step_gvox_s": round(n * steps / tr / 1e9, 4),
     }
+
+
+def _edc_db(ir):
+    e = np.cumsum(ir[::-1].astype(np.float64) ** 2)[::-1]
+    return 10 * np.log10(np.maximum(e / e[0], 1e-300))
+
+
+def precision_error():
+    """Storage precisions against the FP64 run of the same scheme on the
+    5x4x3 m shoebox (160x132x108, 2000 steps, 8 receivers): the largest
+    relative L2 IR error and the largest energy-decay-curve deviation (dB,
+    above -60 dB) over the receivers."""
+    from virs_amd import rooms
+    from virs_amd.fdtd import setup
+    prep = rooms.prepare(rooms.shoebox_5x4x3(receivers=8))
+    irs = {}
+    for prec, rnd in (("fp64", "stochastic"), ("fp32", "stochastic"), ("fp16", "stochastic"),
+                      ("bf16", "stochastic"), ("fp16", "nearest")):
+        f = setup(prep, prec, rounding=rnd)
+        f.step(prep.scenario.steps)
+        irs[prec if rnd == "stochastic" else prec + "_nearest"] = f.receivers(True)
+        f.close()
+    ref = irs.pop("fp64")
+    out = {"workload": "shoebox 5x4x3 m, 160x132x108, 2000 steps, 8 receivers, vs FP64 run"}
+    for k, ir in irs.items():
+        rel = max(float(np.linalg.norm(ir[r] - ref[r]) / np.linalg.norm(ref[r])) for r in range(len(ref)))
+        dev = 0.0
+        for r in range(len(ref)):
+            a, b = _edc_db(ir[r]), _edc_db(ref[r])
+            keep = b > -60.0
+            dev = max(dev, float(np.max(np.abs(a[keep] - b[keep]))))
+        out[k] = {"rel_l2_max": float(f"{rel:.3e}"), "edc_dev_db_max": round(dev, 4)}
+    return out
 
 
 def adjoint_cost(steps=4096, ncp=64):
@@ -235,6 +271,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch every step individually")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
+    ap.add_argument("--no-precision", action="store_true", help="skip the storage-precision error runs")
     ap.add_argument("--large", action="store_true",
                     help="only the 1536x1024x768 FP16 room and the FP32 adjoint on its 512^3 crop")
     ap.add_argument("--adj-steps", type=int, default=40000, help="steps of the --large crop adjoint")
@@ -323,6 +360,8 @@ def main():
                           "dense_ms": round(rv["dense_ms"], 5), "boundary_kernel_ms": round(rv["bnd_ms"], 5)}
     if variants:
         out["variants"] = variants
+    if not args.no_precision:
+        out["precision_error"] = precision_error()
     if not args.no_adjoint:
         out["adjoint"] = adjoint_cost()
     if not args.no_cpu:

@@ -4,13 +4,12 @@
    arithmetic, nearest-even or stochastic storage rounding) against the
    oracle's restatement of the same storage rules, on small grids.
 2. Accuracy against the FP64 run of the same scheme on the 5x4x3 m shoebox
-   with 8 receivers (2000 steps).  Measured: FP32 ~1e-6 relative L2; FP16
-   with stochastic rounding ~2.6-4.5e-3 (a stationary noise floor: the
-   per-store rounding noise times the room's decay time, mostly above 0.2 of
-   Nyquist); FP16 with nearest-even rounding ~2-4e-2 (its deadband loses the
-   small per-step increments and the room's DC level drifts); BF16 ~3e-2
-   stochastic, ~1e-1 nearest.  The 1e-3 goal for FP16 is not met; the
-   bounds below pin the measured levels.
+   with 8 receivers (2000 steps); the largest relative L2 error over the
+   receivers, measured: FP32 1.9e-5; FP16 4.5e-3 with stochastic rounding,
+   5.4e-3 with nearest-even rounding; BF16 3.4e-2 (stochastic).  The FP16
+   error is a noise floor of the 11-bit significand: the per-store rounding
+   noise accumulates over the room's decay time.  The 1e-3 goal for FP16 is
+   not met; the bounds below pin the measured levels.
 """
 import numpy as np
 import pytest
@@ -73,7 +72,7 @@ def test_fp32_vs_fp64(shoebox_runs):
     assert max(errs) <= 1e-4
 
 
-@pytest.mark.parametrize("key,bound", [("fp16", 6e-3), ("fp16_rne", 6e-2), ("bf16", 5e-2)])
+@pytest.mark.parametrize("key,bound", [("fp16", 6e-3), ("fp16_rne", 1e-2), ("bf16", 5e-2)])
 def test_16bit_error_vs_fp64(shoebox_runs, key, bound):
     ref = shoebox_runs["fp64"]
     run = shoebox_runs[key]
