@@ -113,6 +113,9 @@ def host():
     _sig(L, "virs_boundary_materials", C.c_int, [c_u8p, c_u8p, c_i64p, c_u32p, C.c_int64, c_u8p])
     _sig(L, "virs_rbj_highpass", C.c_int, [C.c_float, C.c_float, C.c_float, C.c_float, c_fp])
     _sig(L, "virs_rbj_lowpass", C.c_int, [C.c_float, C.c_float, C.c_float, C.c_float, c_fp])
+    _sig(L, "virs_ir_resample", C.c_int, [c_fp, C.c_int64, C.c_double, C.c_double, c_fp, C.c_int64])
+    _sig(L, "virs_ir_audio", C.c_int, [c_fp, C.c_int64, C.c_float, C.c_int32, c_fp, c_i64p])
+    _sig(L, "virs_wav_write_f32", C.c_int, [C.c_char_p, c_fp, C.c_int64, C.c_int32])
     _host = L
     return L
 
@@ -132,4 +135,5 @@ HOST_SYMBOLS = [
     "virs_mesh_bounds", "virs_mesh_free", "virs_grid_from_bounds", "virs_reference_cell_size",
     "virs_voxelize", "virs_build_shell", "virs_compute_normals", "virs_boundary_list",
     "virs_boundary_materials", "virs_rbj_highpass", "virs_rbj_lowpass",
+    "virs_ir_resample", "virs_ir_audio", "virs_wav_write_f32",
 ]

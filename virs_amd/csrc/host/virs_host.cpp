@@ -482,4 +482,124 @@ int virs_rbj_lowpass(float fs, float fc, float q, float gain, float out[5])
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// IR post-processing of Simulator::simulate (simulator.cpp:1029-1082): the
+// simulation-rate IR is resampled to the audio rate, normalised to a peak of
+// one and written as a mono 32-bit float WAV.  The reference's resampler
+// (kfr 6.3.0, resample_quality::high) is not available here; this one is a
+// Kaiser-windowed sinc evaluated at every output instant, with zero phase
+// (no added delay), passband to 0.833 of the lower Nyquist frequency and a
+// stop band from that Nyquist frequency on, attenuated by about 110 dB.
+
+static double bessel_i0(double x)
+{
+    // power series of the modified Bessel function I0
+    double sum = 1.0, term = 1.0;
+    const double q = 0.25 * x * x;
+    for (int k = 1; k < 200; ++k) {
+        term *= q / ((double)k * (double)k);
+        sum += term;
+        if (term < sum * 1e-17) break;
+    }
+    return sum;
+}
+
+int virs_ir_resample(const float *in, int64_t n_in, double fs_in, double fs_out, float *out, int64_t n_out)
+{
+    if (n_in < 0 || n_out < 0 || (n_in && !in) || (n_out && !out) || !(fs_in > 0.0) || !(fs_out > 0.0))
+        return -1;
+    const double band = 0.5 * std::min(fs_in, fs_out);     // Nyquist frequency of the slower rate
+    const double fc = 0.9167 * band / fs_in;                // cutoff, cycles per input sample
+    const double df = 0.1667 * band / fs_in;                // transition width, cycles per input sample
+    const double atten = 110.0;                             // dB
+    const double beta = 0.1102 * (atten - 8.7);             // Kaiser's window parameter for atten
+    const double half = (atten - 8.0) / (2.285 * 2.0 * M_PI * df) / 2.0; // half-length, input samples
+    const double i0b = bessel_i0(beta);
+    const double step = fs_in / fs_out;                     // input samples per output sample
+    for (int64_t k = 0; k < n_out; ++k) {
+        const double t = (double)k * step;
+        const int64_t j0 = std::max<int64_t>(0, (int64_t)std::ceil(t - half));
+        const int64_t j1 = std::min<int64_t>(n_in - 1, (int64_t)std::floor(t + half));
+        double acc = 0.0, wsum = 0.0;
+        // the taps are normalised by their sum over the whole window, so a
+        // constant input gives exactly that constant away from the ends
+        const int64_t a0 = (int64_t)std::ceil(t - half), a1 = (int64_t)std::floor(t + half);
+        for (int64_t j = a0; j <= a1; ++j) {
+            const double x = t - (double)j;
+            const double r = x / half;
+            const double w = bessel_i0(beta * std::sqrt(std::max(0.0, 1.0 - r * r))) / i0b;
+            const double arg = 2.0 * fc * x;
+            const double sinc = std::fabs(arg) < 1e-12 ? 1.0 : std::sin(M_PI * arg) / (M_PI * arg);
+            const double hv = 2.0 * fc * sinc * w;
+            wsum += hv;
+            if (j >= j0 && j <= j1) acc += hv * (double)in[j];
+        }
+        out[k] = (float)(wsum != 0.0 ? acc / wsum : 0.0);
+    }
+    return 0;
+}
+
+int virs_wav_write_f32(const char *path, const float *samples, int64_t n, int32_t rate)
+{
+    if (!path || n < 0 || (n && !samples) || rate <= 0 || n > (int64_t)(0xffffffffu - 64) / 4) return -1;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return -2;
+    auto u32 = [&](uint32_t v) {
+        const unsigned char b[4] = {(unsigned char)v, (unsigned char)(v >> 8), (unsigned char)(v >> 16),
+                                    (unsigned char)(v >> 24)};
+        std::fwrite(b, 1, 4, f);
+    };
+    auto u16 = [&](uint16_t v) {
+        const unsigned char b[2] = {(unsigned char)v, (unsigned char)(v >> 8)};
+        std::fwrite(b, 1, 2, f);
+    };
+    const uint32_t data = (uint32_t)(4 * n);
+    std::fwrite("RIFF", 1, 4, f);
+    u32(4 + (8 + 16) + (8 + 4) + (8 + data));
+    std::fwrite("WAVE", 1, 4, f);
+    std::fwrite("fmt ", 1, 4, f);
+    u32(16);
+    u16(3); // IEEE float
+    u16(1); // mono
+    u32((uint32_t)rate);
+    u32((uint32_t)rate * 4u);
+    u16(4);
+    u16(32);
+    std::fwrite("fact", 1, 4, f); // sample count, as non-PCM formats carry
+    u32(8 - 4);
+    u32((uint32_t)n);
+    std::fwrite("data", 1, 4, f);
+    u32(data);
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t v;
+        std::memcpy(&v, samples + i, 4);
+        u32(v);
+    }
+    const bool ok = std::ferror(f) == 0;
+    return (std::fclose(f) == 0 && ok) ? 0 : -3;
+}
+
+int virs_ir_audio(const float *ir, int64_t steps, float h, int32_t rate, float *out, int64_t *n_out)
+{
+    if (!n_out || steps < 0 || (steps && !ir) || !(h > 0.f) || rate <= 0) return -1;
+    // simulator.cpp:930-933,1031-1052: dt and the sample count in double,
+    // the simulation rate truncated to whole Hz for the resampler
+    const double dt = 0.5 * (double)h / (343.0 * std::sqrt(3.0));
+    const double fs_sim = (double)(size_t)(1.f / dt);
+    const int64_t n = (int64_t)(size_t)((double)rate * ((double)steps * dt));
+    if (!out) {
+        *n_out = n;
+        return 0;
+    }
+    if (*n_out < n) return -1;
+    int rc = virs_ir_resample(ir, steps, fs_sim, (double)rate, out, n);
+    if (rc) return rc;
+    float peak = 0.f;
+    for (int64_t i = 0; i < n; ++i) peak = std::max(peak, std::fabs(out[i]));
+    if (peak > 0.f)
+        for (int64_t i = 0; i < n; ++i) out[i] /= peak;
+    *n_out = n;
+    return 0;
+}
+
 } // extern "C"
