@@ -109,7 +109,12 @@ int virs_set_soft_sources(virs_ctx *ctx, const int64_t *idx, const double *sig, 
 /* Receiver voxels.  The reference has one, fixed at (334,225,270)
  * (simulator_gpu.cu:295). */
 int virs_set_receivers(virs_ctx *ctx, const int64_t *idx, int32_t nrcv);
-/* RMS early stop of simulator_gpu.cu:303-330 on receiver 0; off by default. */
+/* RMS early stop of simulator_gpu.cu:303-330 on receiver 0; off by default.
+ * The device evaluates the rule at the end of every step and skips every
+ * step after the one where it fires, also inside graph replays; virs_step
+ * reads the verdict once a step past 3000 has run and then leaves the
+ * fields at the step right after the one that fired.  It is off while
+ * virs_time_steps, virs_forward_record and virs_adjoint run. */
 int virs_set_early_stop(virs_ctx *ctx, int32_t enable, float target_rms);
 
 /* fdtd_start_simulation (simulator_dispatch.cpp:93-100): size the receiver

@@ -147,3 +147,40 @@ def test_shoebox_2000_steps_vs_reference_scheme(gpu):
     _say(f"shoebox fp32 vs reference-literal rel L2 = {err:.3e}")
     assert err <= 1e-4
     np.testing.assert_array_equal(ir, same)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_device_rms_stop_in_one_batch(gpu, use_graph):
+    """The RMS rule (simulator_gpu.cu:303-330) runs on the device: a single
+    3400-step call, replayed as graphs or launched step by step, stops at the
+    oracle's step, and the steps after it are skipped on the device (the
+    final field is the one right after the stopping step)."""
+    from virs_amd import grid as G
+    from virs_amd.fdtd import FDTD
+    p = tiny_problem(2, dims=(28, 20, 16))
+    flags = p["flags"]
+    nz, ny, nx = flags.shape
+    h, steps = 0.05, 3400
+    dt = 0.5 * h / (343.0 * np.sqrt(3.0))
+    row = G.rbj_highpass(np.float32(1.0 / dt), 440.0, 0.1, 1.0).astype(np.float64)[None]
+    ridx = ((nz // 2) * ny + ny // 2 + 2) * nx + nx // 2
+    bidx = G.boundary_list(flags)
+    pb = O.Problem(flags, h, bidx=bidx, coeffs=row, rcv_idx=[ridx])
+    ir = pb.run(steps, "f32")["ir"][0].astype(np.float32)
+    stop = O.rms_continue_sequence(ir)
+    assert 3000 < stop < steps - 100
+    _say(f"device RMS stop graph={use_graph}, oracle stop {stop}")
+    f = FDTD((nx, ny, nz), h, use_graph=use_graph)
+    f.upload_flags(flags)
+    f.upload_boundary(bidx)
+    f.set_materials(row)
+    f.set_receivers([ridx])
+    f.set_early_stop(True)
+    f.start(steps)
+    assert f.step(steps) is False
+    assert f.steps_done == stop + 1
+    np.testing.assert_array_equal(f.receivers()[0], ir[: stop + 1])
+    after = pb.run(stop + 1, "f32", want_fields=True)
+    np.testing.assert_array_equal(f.field(0), after["p"].astype(np.float32))
+    assert f.percent_to_target_rms > 1.0
+    f.close()

@@ -34,12 +34,25 @@ constexpr int kMaxPoints = 64;
 // floats order like their bit patterns).  The boundary correction, a bounded
 // damping term, and the point sources, which enter the exponent's estimate
 // themselves, do not contribute to it.
+// The RMS early stop on receiver 0 (simulator_gpu.cu:303-330) is evaluated
+// on the device at the end of every step.  The step that fires records
+// itself in stop_at and hands the next step a counter with kStopBit set;
+// every launch that reads a counter with the bit exits at once (the list
+// launch still passing the counter on), so batches and graph replays stop
+// without a host round trip and without a load beyond the counter itself.
+constexpr int kRmsWindow = 100, kRmsMinStep = 3000;
+constexpr long long kStopBit = 1ll << 62;
 struct StepCtl {
     long long nring[3];
     int err;
     int pad;
     int exp[4];
     unsigned int maxabs[4][kSlots];
+    int es_enable;
+    int pad1;
+    long long stop_at;   // LLONG_MAX until the rule fires
+    float es_target, rms_state, percent;
+    int pad2;
 };
 
 // 16-bit float storage (IEEE binary16) and bfloat16 storage.
@@ -485,6 +498,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
     const unsigned logical = (nbk % 8u == 0u) ? (bid % 8u) * (nbk / 8u) + bid / 8u : bid;
 
     const long long n = p.ctl->nring[p.slot];
+    if (n & kStopBit) return;
     C hard = (C)0;
     if (!ADJ) {
         if (p.hard_ref) hard = n < 10 ? (C)10 : (C)0;
@@ -919,6 +933,11 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
     using C = typename S::C;
     constexpr int U = tail_batch<T>();
     const long long n = p.ctl->nring[p.slot];
+    if (n & kStopBit) {
+        // a skipped step still passes the counter (and its stop bit) on
+        if (blockIdx.x == 0 && threadIdx.x == 0) p.ctl->nring[p.next_slot] = n + 1;
+        return;
+    }
     C sP = (C)1, sQ = (C)1, sR = (C)1, sRi = (C)1;
     if constexpr (S::kScaled) {
         sP = exp2i(-p.ctl->exp[(n - 1) & 3]);
@@ -998,7 +1017,34 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
             if constexpr (S::kScaled) r = r * sRi;
             p.ir[(long long)k * p.ir_cap + n] = (double)r;
         }
-        if (k == 0) p.ctl->nring[p.next_slot] = n + 1;
+        // RMS rule of step n over receiver 0's samples [n-100, n), in float
+        // in the reference's order (simulator_gpu.cu:303-330)
+        StepCtl *ctl = p.ctl;
+        long long next = n + 1;
+        if (ctl->es_enable && p.nrcv > 0 && n > kRmsMinStep && n <= p.ir_cap) {
+            __shared__ float win[kRmsWindow];
+            if (k < kRmsWindow) win[k] = (float)p.ir[n - kRmsWindow + k];
+            __syncthreads();
+            if (k == 0) {
+                float sum = 0.0f;
+                for (int i = 0; i < kRmsWindow; ++i) sum += win[i] * win[i];
+                float rms = __fsqrt_rn(sum / (float)kRmsWindow);
+                if (n == kRmsMinStep + 1) {
+                    ctl->rms_state = rms;
+                } else {
+                    const float fg = 0.00008f;
+                    const float vout = (fg * rms + ctl->rms_state) / (fg + 1.f);
+                    ctl->rms_state = 2.f * vout - ctl->rms_state;
+                    rms = vout;
+                }
+                ctl->percent = ctl->es_target / rms;
+                if (rms < ctl->es_target) {
+                    ctl->stop_at = n;
+                    next |= kStopBit;
+                }
+            }
+        }
+        if (k == 0) ctl->nring[p.next_slot] = next;
     }
 }
 
@@ -1141,7 +1187,8 @@ __global__ void k_from_f64(const double *__restrict__ in, T *__restrict__ out, l
 }
 
 // Sets the step index of rotation phase `slot` (and, with reset, clears the
-// control block) without a memset, so it can sit inside a captured graph.
+// control block's step state; the early-stop settings stay) without a
+// memset, so it can sit inside a captured graph.
 __global__ void k_ctl_set(StepCtl *ctl, int slot, long long n, int reset)
 {
     const int t = threadIdx.x;
@@ -1150,9 +1197,23 @@ __global__ void k_ctl_set(StepCtl *ctl, int slot, long long n, int reset)
             ctl->exp[t] = 0;
             for (int s = 0; s < kSlots; ++s) ctl->maxabs[t][s] = 0u;
         }
-        if (t == 0) ctl->err = 0;
+        if (t == 0) {
+            ctl->err = 0;
+            ctl->stop_at = 0x7fffffffffffffffll;
+            ctl->rms_state = 0.f;
+            ctl->percent = 0.f;
+        }
     }
     if (t == 0) ctl->nring[slot] = n;
+}
+
+// Early-stop settings of the control block.
+__global__ void k_ctl_es(StepCtl *ctl, int enable, float target)
+{
+    if (threadIdx.x == 0) {
+        ctl->es_enable = enable;
+        ctl->es_target = target;
+    }
 }
 
 // Copies the filter state at a segment start into trace slots 0 (step s0-2)

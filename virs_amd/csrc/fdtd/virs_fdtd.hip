@@ -77,8 +77,7 @@ struct virs_ctx {
     long long counted = 0; // steps reported (early stop truncates)
     bool stopped = false;
     int es_enable = 0;
-    float es_target = 0.0009f, rms_state = 0.f, percent = 0.f;
-    std::vector<float> ir0_host;
+    float es_target = 0.0009f, percent = 0.f;
 
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
     int graph_steps = 0;
@@ -516,8 +515,7 @@ int reset_run(virs_ctx *c)
     if (rc) return rc;
     c->steps = c->counted = 0;
     c->stopped = false;
-    c->rms_state = c->percent = 0.f;
-    c->ir0_host.clear();
+    c->percent = 0.f;
     drop_graphs(c);
     return VIRS_OK;
 }
@@ -612,6 +610,12 @@ int virs_create(const virs_desc *d, virs_ctx **out)
     if (!rc) rc = dalloc(c, &c->neg, sizeof(long long) * kMaxPoints);
     if (!rc && hipMemset(c->neg, 0xff, sizeof(long long) * kMaxPoints) != hipSuccess) rc = VIRS_ERR_HIP;
     if (!rc) rc = reset_run(c);
+    if (!rc) {
+        hipLaunchKernelGGL(k_ctl_es, dim3(1), dim3(64), 0, c->stream, c->ctl, 0, c->es_target);
+        hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl_adj, 0, 0LL, 1);
+        hipLaunchKernelGGL(k_ctl_es, dim3(1), dim3(64), 0, c->stream, c->ctl_adj, 0, c->es_target);
+        if (hipGetLastError() != hipSuccess) rc = VIRS_ERR_HIP;
+    }
     if (!rc && hipMemsetAsync(c->code, kTypeZero << 6, (size_t)c->N, c->stream) != hipSuccess) rc = VIRS_ERR_HIP;
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = VIRS_ERR_HIP;
     if (rc) {
@@ -847,6 +851,18 @@ int virs_set_early_stop(virs_ctx *c, int32_t enable, float target)
     if (!c) return VIRS_ERR_ARG;
     c->es_enable = enable;
     c->es_target = target > 0.f ? target : 0.0009f;
+    HIPC(c, hipSetDevice(c->d.device));
+    hipLaunchKernelGGL(k_ctl_es, dim3(1), dim3(64), 0, c->stream, c->ctl, c->es_enable, c->es_target);
+    HIPC(c, hipGetLastError());
+    return VIRS_OK;
+}
+
+// Switches the device-side RMS rule off (timing, recording, adjoint) or
+// back to the context's setting.
+static int device_early_stop(virs_ctx *c, bool on)
+{
+    hipLaunchKernelGGL(k_ctl_es, dim3(1), dim3(64), 0, c->stream, c->ctl, on ? c->es_enable : 0, c->es_target);
+    HIPC(c, hipGetLastError());
     return VIRS_OK;
 }
 
@@ -921,42 +937,34 @@ int virs_step(virs_ctx *c, int64_t nsteps, int32_t *keep_going)
         ++c->steps;
     }
     c->counted = c->steps;
-    const unsigned RMS_SIZE = 100, MIN_STEP = 3000;
-    if (c->es_enable && c->nrcv > 0 && c->steps > (long long)MIN_STEP + 1) {
-        // RMS rule of simulator_gpu.cu:303-330 on receiver 0, in float.  No
-        // step up to MIN_STEP can stop the run, so the samples are fetched
-        // (and the stream waited on) only from then on.
-        const long long have = (long long)c->ir0_host.size();
-        const long long last = std::min<long long>(c->steps, c->ir_cap);
-        if (last > have) {
-            std::vector<double> tmp((size_t)(last - have));
-            HIPC(c, hipMemcpyAsync(tmp.data(), c->ir_buf + have, sizeof(double) * (last - have),
-                                   hipMemcpyDeviceToHost, c->stream));
-            HIPC(c, hipStreamSynchronize(c->stream));
-            for (double v : tmp) c->ir0_host.push_back((float)v);
-        }
-        for (long long s = first; s < last; ++s) {
-            const unsigned step = (unsigned)s;
-            if (step > MIN_STEP) {
-                float sum = 0.0f;
-                for (unsigned i = step - RMS_SIZE; i < step; ++i) sum += c->ir0_host[i] * c->ir0_host[i];
-                float rms = std::sqrt(sum / (float)RMS_SIZE);
-                if (step == MIN_STEP + 1) {
-                    c->rms_state = rms;
-                } else {
-                    const float fg = 0.00008f;
-                    const float vout = (fg * rms + c->rms_state) / (fg + 1.f);
-                    c->rms_state = 2.f * vout - c->rms_state;
-                    rms = vout;
-                }
-                c->percent = c->es_target / rms;
-                if (rms < c->es_target) {
-                    c->stopped = true;
-                    c->counted = s + 1;
-                    if (keep_going) *keep_going = 0;
-                    break;
-                }
+    if (c->es_enable && c->nrcv > 0 && c->steps > (long long)kRmsMinStep + 1) {
+        // The device evaluates the RMS rule at the end of every step and
+        // skips every step after the one where it fires; read its verdict.
+        // No step up to kRmsMinStep can fire, so the stream is waited on
+        // only from then on.
+        long long stop_at = 0;
+        float percent = 0.f;
+        HIPC(c, hipMemcpyAsync(&stop_at, &c->ctl->stop_at, sizeof stop_at, hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipMemcpyAsync(&percent, &c->ctl->percent, sizeof percent, hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        c->percent = percent;
+        if (stop_at != 0x7fffffffffffffffll) {
+            // the steps launched after stop_at left the fields untouched:
+            // rewind the rotation to just after stop_at and point its slot
+            // of the step counter back at stop_at + 1, clearing the stop bit
+            const long long extra = c->steps - (stop_at + 1);
+            for (long long k = 0; k < extra % 3; ++k) {
+                const int ip = c->ip;
+                c->ip = c->iq;
+                c->iq = c->ir;
+                c->ir = ip;
             }
+            c->steps = c->counted = stop_at + 1;
+            hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl, c->ip, c->steps, 0);
+            HIPC(c, hipGetLastError());
+            HIPC(c, hipStreamSynchronize(c->stream));
+            c->stopped = true;
+            if (keep_going) *keep_going = 0;
         }
     }
     return VIRS_OK;
@@ -1168,6 +1176,8 @@ int virs_time_steps(virs_ctx *c, int64_t nsteps, double *total_ms, double *dense
         rc = ensure_graphs(c, gsteps);
         if (rc) return rc;
     }
+    rc = device_early_stop(c, false); // every timed step does its work
+    if (rc) return rc;
     const long long stride = graphs ? gsteps + 1 : std::max<long long>(1, nsteps / 64);
     const long long nsamp = (nsteps + stride - 1) / stride;
     std::vector<hipEvent_t> ev((size_t)nsamp * 3);
@@ -1200,6 +1210,7 @@ int virs_time_steps(virs_ctx *c, int64_t nsteps, double *total_ms, double *dense
     }
     c->counted = c->steps;
     if (!rc) HIPC(c, hipEventRecord(t1, c->stream));
+    if (!rc) rc = device_early_stop(c, true);
     if (!rc) HIPC(c, hipEventSynchronize(t1));
     double dsum = 0, bsum = 0;
     for (long long j = 0; j < k && !rc; ++j) {
@@ -1319,6 +1330,9 @@ static int recompute_segment(virs_ctx *c, int k)
     return rc;
 }
 
+// The RMS stop stays off while a run is recorded for the adjoint and while
+// the adjoint recomputes it: every recorded step is kept.  on = 1 restores
+// the context's setting.
 int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
 {
     int rc = check_ready(c);
@@ -1347,7 +1361,8 @@ int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
     c->ncp = ncp;
     rc = ensure_trace(c, c->seg + 2);
     if (rc) return rc;
-    rc = reset_run(c);
+    rc = device_early_stop(c, false);
+    if (!rc) rc = reset_run(c);
     if (rc) return rc;
     const long long last_base = (long long)(ncp - 1) * c->seg;
     const int last_buf = (ncp - 1) & 1;
@@ -1364,9 +1379,12 @@ int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
     }
     c->counted = c->steps;
     c->rec_steps = nsteps;
+    rc = device_early_stop(c, true);
+    if (rc) return rc;
     HIPC(c, hipStreamSynchronize(c->stream));
     return VIRS_OK;
 }
+
 
 // Reverse steps of segment k on the current launch stream.
 static int reverse_segment(virs_ctx *c, int k, int &ap, int &aq, int &ar)
@@ -1438,6 +1456,8 @@ int virs_adjoint(virs_ctx *c, const float *dl_dir, double *grad)
         if (c->nb) HIPC(c, hipMemsetAsync(c->ast[k], 0, sizeof(double) * c->nb, c->stream));
     hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl_adj, 0, T - 1, 1);
     HIPC(c, hipGetLastError());
+    rc = device_early_stop(c, false);
+    if (rc) return rc;
     // the second stream starts after everything queued so far
     HIPC(c, hipStreamSynchronize(c->stream));
     int ap = 0, aq = 1, ar = 2;
@@ -1472,6 +1492,9 @@ int virs_adjoint(virs_ctx *c, const float *dl_dir, double *grad)
     // the forward state is left as after a finished run; a new
     // virs_forward_record is needed before the next adjoint
     c->rec_steps = 0;
+    rc = device_early_stop(c, true);
+    if (rc) return rc;
+    HIPC(c, hipStreamSynchronize(c->stream));
     return VIRS_OK;
 }
 
