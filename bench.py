@@ -258,6 +258,55 @@ def large_room(steps=1000, warmup=20, adj_steps=40000, ncp=200):
     return out
 
 
+def layer_images(prep, steps=500, interval=10, precision="fp32"):
+    """simulate's loop with a layer image every `interval` steps (the
+    reference's default image_save_interval) against the same steps without
+    images, wall clock from virs_start_simulation on, 30-step graphs.  The
+    image loop's time includes its last PNG.  The images go to a temporary
+    directory; png_ms is one image's rendering and encoding on the host,
+    snapshot_ms one plane request + fetch on an idle GPU.  The first
+    snapshot of a context allocates the snapshot ring; it is timed apart."""
+    import tempfile
+    from virs_amd import layers
+    from virs_amd.fdtd import setup
+    f = setup(prep, precision, use_graph=True)
+    flags = prep.room.flags
+    y = int(prep.room.dims[1]) // 2
+    f.start(steps)
+    f.step(min(steps, 40))
+    f.synchronize()
+    t0 = time.perf_counter()
+    f.plane_fetch(f.plane_request(y))  # allocates the snapshot ring (once per context)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    f.start(steps)
+    f.step(steps)
+    f.synchronize()
+    t_plain = time.perf_counter() - t0
+    with tempfile.TemporaryDirectory() as d:
+        t0 = time.perf_counter()
+        paths = layers.simulate_with_layer_images(f, flags, steps, y, interval, out_dir=d)
+        f.synchronize()
+        t_img = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(5):
+            plane = f.plane_fetch(f.plane_request(y))
+        t_snap = (time.perf_counter() - t0) / 5
+        t0 = time.perf_counter()
+        for _ in range(5):
+            layers.write_png(os.path.join(d, "x.png"), layers.layer_rgb(plane, flags, y))
+        t_png = (time.perf_counter() - t0) / 5
+    nx, ny, nz = (int(v) for v in prep.room.dims)
+    f.close()
+    return {"workload": f"random room, {precision}, {steps} steps, XZ plane y={y} every {interval} steps",
+            "images": len(paths), "ms_per_step_plain": round(t_plain / steps * 1e3, 5),
+            "ms_per_step_with_images": round(t_img / steps * 1e3, 5),
+            "overhead_frac": round(t_img / t_plain - 1.0, 4),
+            "snapshot_ms": round(t_snap * 1e3, 4), "png_ms": round(t_png * 1e3, 3),
+            "first_snapshot_setup_ms": round(t_setup * 1e3, 3),
+            "plane_bytes_per_image": nx * nz * 4, "reference_bytes_per_image": nx * ny * nz * 4}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,6 +321,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch every step individually")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
     ap.add_argument("--no-precision", action="store_true", help="skip the storage-precision error runs")
+    ap.add_argument("--no-images", action="store_true", help="skip the layer-image loop measurement")
     ap.add_argument("--large", action="store_true",
                     help="only the 1536x1024x768 FP16 room and the FP32 adjoint on its 512^3 crop")
     ap.add_argument("--adj-steps", type=int, default=40000, help="steps of the --large crop adjoint")
@@ -364,6 +414,8 @@ def main():
         out["precision_error"] = precision_error()
     if not args.no_adjoint:
         out["adjoint"] = adjoint_cost()
+    if not args.no_images:
+        out["layer_images"] = layer_images(prep)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prep)
     print(json.dumps(out), flush=True)

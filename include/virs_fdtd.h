@@ -137,6 +137,17 @@ int virs_read_receivers_f64(virs_ctx *ctx, double *out);
  * the reference's p_next after its pointer rotation); values in physical units. */
 int virs_read_field(virs_ctx *ctx, int32_t which, float *out);
 int virs_read_field_f64(virs_ctx *ctx, int32_t which, double *out);
+/* Layer snapshot: the part of updateCurrentGridFromGPU that layerToImage
+ * (vectorspace.cpp:77-148, called from simulator.cpp:975-1018) reads, i.e.
+ * the XZ plane y = layer of the current field p^n, instead of the whole
+ * field.  virs_plane_request queues a gather of the plane and its copy into
+ * pinned host memory behind the steps already launched and returns at once
+ * with a ticket; virs_plane_fetch waits for that snapshot alone and writes
+ * out[z*Nx + x] (Nz*Nx floats, the values virs_read_field gives).  At most 4
+ * snapshots are outstanding; a fifth request, or a layer outside [0, Ny),
+ * fails with VIRS_ERR_ARG. */
+int virs_plane_request(virs_ctx *ctx, int64_t layer, int32_t *ticket);
+int virs_plane_fetch(virs_ctx *ctx, int32_t ticket, float *out);
 /* updateGPUFromGrid, field part: set p^n (which 0) or p^{n-1} (which 1). */
 int virs_write_field_f64(virs_ctx *ctx, int32_t which, const double *in);
 /* Power-of-two exponent e of a stored field (which as for virs_read_field;

@@ -106,6 +106,24 @@ int virs_ir_resample(const float *in, int64_t n_in, double fs_in, double fs_out,
 int virs_ir_audio(const float *ir, int64_t steps, float h, int32_t rate, float *out, int64_t *n_out);
 int virs_wav_write_f32(const char *path, const float *samples, int64_t n, int32_t rate);
 
+/*
+ * Layer images (VectorSpace::layerToImage, vectorspace.cpp:77-148).
+ * virs_layer_rgb: the reference's pixels for the XZ plane y = layer, from
+ * plane[z*Nx + x] (virs_plane_fetch) and the full flag grid.  The image is
+ * (Nx rounded up to even) x (Nz rounded up to even) RGB bytes, *width and
+ * *height; with rgb == NULL only the size is set.  Per cell: G = 100 for
+ * flag 1, 255 for flag 2, else 0; R = clamp(p*1500, 0, 255) for p >= 0, else
+ * B = clamp(-p*1500, 0, 255), truncated to bytes.  As in the reference,
+ * pixel (x, z) sits at byte 3*(z*Nx + x) whatever the padded width, so an
+ * odd Nx shears the picture, and the padding is black.  Returns -2 for a
+ * layer outside [0, Ny), as the reference throws std::out_of_range.
+ * virs_png_write_rgb: 8-bit RGB PNG of width x height pixels, rows of
+ * 3*width bytes (save_png, util/image.cpp:8-57).  The pixels match; the
+ * compressed bytes are this writer's own (zlib, no row filters). */
+int virs_layer_rgb(const float *plane, const uint8_t *flags, int64_t nx, int64_t ny, int64_t nz, int64_t layer,
+                   uint8_t *rgb, int32_t *width, int32_t *height);
+int virs_png_write_rgb(const char *path, const uint8_t *rgb, int32_t width, int32_t height);
+
 #ifdef __cplusplus
 }
 #endif

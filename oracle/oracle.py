@@ -244,6 +244,52 @@ def rms_continue_sequence(ir, target=0.0009):
     return -1
 
 
+def layer_image(p_curr, flags, layer):
+    """VectorSpace::layerToImage (src/simulator/vectorspace.cpp:77-148) as the
+    reference writes it: loops over the padded (even) width and height, the
+    padded cells written black at their padded-stride position, the others at
+    k*width + i.  Pure-Python loops in float32, for small grids.  Returns the
+    byte buffer as uint8 [H, W, 3] and raises IndexError for a layer outside
+    [0, Ny) (the reference throws std::out_of_range, :79-80)."""
+    flags = np.asarray(flags, np.uint8)
+    nz, ny, nx = flags.shape
+    if not 0 <= layer < ny:
+        raise IndexError("layerToImage: layer index out of range")
+    p = np.asarray(p_curr, np.float32).reshape(-1)
+    fl = flags.reshape(-1)
+    width, height = nx, nz
+    ow = width + 1 if width % 2 else width
+    oh = height + 1 if height % 2 else height
+    scale = f32(1500.0)
+    px = [0] * (ow * oh * 3)
+
+    def clamp_byte(v):  # std::clamp(v, 0, 255) then the unsigned char conversion
+        v = f32(v)
+        v = f32(0) if v < 0 else (f32(255) if v > 255 else v)
+        return int(v)
+
+    for k in range(oh):
+        row = k * nx * ny + layer * nx
+        for i in range(ow):
+            if i >= width or k >= height:
+                px[(k * ow + i) * 3: (k * ow + i) * 3 + 3] = [0, 0, 0]
+                continue
+            v = p[row + i]
+            flag = int(fl[row + i])
+            red = green = blue = 0
+            if flag == 1:
+                green = 100
+            if flag == 2:
+                green = 255
+            if v >= f32(0):
+                red = clamp_byte(v * scale)
+            else:
+                blue = clamp_byte(-v * scale)
+            j = (k * width + i) * 3
+            px[j: j + 3] = [red, green, blue]
+    return np.array(px, np.uint8).reshape(oh, ow, 3)
+
+
 def refcpu_steps(flags, h, nsteps, p=None, q=None):
     """The reference's own CPU step (simulator_cpu.cpp:55-81) run nsteps."""
     L = lib()

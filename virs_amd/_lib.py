@@ -87,6 +87,8 @@ def hip():
     _sig(L, "virs_time_steps", C.c_int, [vp, C.c_int64, c_dp, c_dp, c_dp])
     _sig(L, "virs_forward_record", C.c_int, [vp, C.c_int64, C.c_int32])
     _sig(L, "virs_adjoint", C.c_int, [vp, c_fp, c_dp])
+    _sig(L, "virs_plane_request", C.c_int, [vp, C.c_int64, c_i32p])
+    _sig(L, "virs_plane_fetch", C.c_int, [vp, C.c_int32, c_fp])
     _hip = L
     return L
 
@@ -116,6 +118,9 @@ def host():
     _sig(L, "virs_ir_resample", C.c_int, [c_fp, C.c_int64, C.c_double, C.c_double, c_fp, C.c_int64])
     _sig(L, "virs_ir_audio", C.c_int, [c_fp, C.c_int64, C.c_float, C.c_int32, c_fp, c_i64p])
     _sig(L, "virs_wav_write_f32", C.c_int, [C.c_char_p, c_fp, C.c_int64, C.c_int32])
+    _sig(L, "virs_layer_rgb", C.c_int, [c_fp, c_u8p, C.c_int64, C.c_int64, C.c_int64, C.c_int64, c_u8p,
+                                        c_i32p, c_i32p])
+    _sig(L, "virs_png_write_rgb", C.c_int, [C.c_char_p, c_u8p, C.c_int32, C.c_int32])
     _host = L
     return L
 
@@ -129,11 +134,12 @@ HIP_SYMBOLS = [
     "virs_percent_to_target_rms", "virs_read_receivers", "virs_read_receivers_f64",
     "virs_read_field", "virs_read_field_f64", "virs_write_field_f64", "virs_field_exponent",
     "virs_memory_bytes", "virs_time_steps", "virs_forward_record", "virs_adjoint",
+    "virs_plane_request", "virs_plane_fetch",
 ]
 HOST_SYMBOLS = [
     "virs_mesh_load_obj", "virs_mesh_from_arrays", "virs_mesh_num_triangles", "virs_mesh_copy",
     "virs_mesh_bounds", "virs_mesh_free", "virs_grid_from_bounds", "virs_reference_cell_size",
     "virs_voxelize", "virs_build_shell", "virs_compute_normals", "virs_boundary_list",
     "virs_boundary_materials", "virs_rbj_highpass", "virs_rbj_lowpass",
-    "virs_ir_resample", "virs_ir_audio", "virs_wav_write_f32",
+    "virs_ir_resample", "virs_ir_audio", "virs_wav_write_f32", "virs_layer_rgb", "virs_png_write_rgb",
 ]

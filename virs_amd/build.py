@@ -48,7 +48,7 @@ def build(verbose=False, out_dir=None, defines=()):
     host_src = os.path.join(PKG, "csrc", "host", "virs_host.cpp")
     hip_out = os.path.join(lib, "libvirs_hip.so")
     host_out = os.path.join(lib, "libvirs_host.so")
-    _run(["g++"] + HOST_FLAGS + [host_src, "-o", host_out + ".tmp"], verbose)
+    _run(["g++"] + HOST_FLAGS + [host_src, "-o", host_out + ".tmp", "-lz"], verbose)
     os.replace(host_out + ".tmp", host_out)
     _run([HIPCC] + HIP_FLAGS + [f"-D{d}" for d in defines] + [hip_src, "-o", hip_out + ".tmp"], verbose)
     os.replace(hip_out + ".tmp", hip_out)

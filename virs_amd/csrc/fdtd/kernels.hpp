@@ -1186,6 +1186,26 @@ __global__ void k_from_f64(const double *__restrict__ in, T *__restrict__ out, l
         out[i] = S::st((C)in[i] * s);
 }
 
+// Packs the XZ plane y = layer of a stored field into out[z*nx + x] as float
+// in physical units, converting exactly as the whole-field readback does
+// (through double, value * 2^-e).  For scaled storage the exponent is read on
+// the device from the step counter of rotation phase `slot`, so the launch
+// can queue behind steps whose exponents the host has not seen.
+template <typename T>
+__global__ void k_plane_y(const T *__restrict__ f, const StepCtl *__restrict__ ctl, int slot,
+                          float *__restrict__ out, int nx, int ny, int nz, int layer)
+{
+    using S = Store<T>;
+    int e = 0;
+    if constexpr (S::kScaled) e = ctl->exp[((ctl->nring[slot] & ~kStopBit) - 1) & 3];
+    const double s = (double)exp2i(-e);
+    for (int z = blockIdx.y; z < nz; z += gridDim.y) {
+        const long long row = ((long long)z * ny + layer) * nx;
+        for (int x = blockIdx.x * blockDim.x + threadIdx.x; x < nx; x += gridDim.x * blockDim.x)
+            out[(long long)z * nx + x] = (float)((double)S::ld(f[row + x]) * s);
+    }
+}
+
 // Sets the step index of rotation phase `slot` (and, with reset, clears the
 // control block's step state; the early-stop settings stay) without a
 // memset, so it can sit inside a captured graph.

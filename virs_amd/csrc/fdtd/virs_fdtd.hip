@@ -19,6 +19,8 @@
 
 using namespace virs;
 
+constexpr int kSnapSlots = 4; // layer snapshots outstanding at once
+
 struct virs_ctx {
     virs_desc d{};
     hipStream_t stream = nullptr;
@@ -99,6 +101,17 @@ struct virs_ctx {
     double *gsig = nullptr;
     double *grad_part = nullptr;
     long long grad_blocks = 0;
+
+    // layer snapshots: kSnapSlots gathered XZ planes on the device, their
+    // pinned host copies, the stream the copies run on (so they overlap the
+    // next steps), gather and copy events; snap_ticket[k] is the ticket held
+    // by slot k (-1 when free), tickets are handed out in order
+    float *snap_dev = nullptr, *snap_host = nullptr;
+    hipStream_t snap_stream = nullptr;
+    hipEvent_t snap_gather[kSnapSlots] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t snap_ev[kSnapSlots] = {nullptr, nullptr, nullptr, nullptr};
+    int snap_ticket[kSnapSlots] = {-1, -1, -1, -1};
+    int snap_next = 0;
 
     std::string err;
 };
@@ -631,6 +644,7 @@ int virs_destroy(virs_ctx *c)
     if (!c) return VIRS_ERR_ARG;
     (void)hipSetDevice(c->d.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->snap_stream) (void)hipStreamSynchronize(c->snap_stream);
     drop_graphs(c);
     for (auto &p : c->f) dfree(p);
     for (auto &p : c->af) dfree(p);
@@ -663,6 +677,13 @@ int virs_destroy(virs_ctx *c)
     }
     dfree(c->gsig);
     dfree(c->grad_part);
+    dfree(c->snap_dev);
+    if (c->snap_host) (void)hipHostFree(c->snap_host);
+    for (auto &e : c->snap_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : c->snap_gather)
+        if (e) (void)hipEventDestroy(e);
+    if (c->snap_stream) (void)hipStreamDestroy(c->snap_stream);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1126,6 +1147,97 @@ int virs_write_field_f64(virs_ctx *c, int32_t which, const double *in)
     return VIRS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Layer snapshots (layerToImage's input, vectorspace.cpp:77-148): one XZ
+// plane of the current field instead of the whole field, gathered on the
+// device and copied into pinned host memory behind the steps already queued.
+// ---------------------------------------------------------------------------
+
+// Allocates the snapshot ring on first use; the pinned buffer comes last, so
+// its presence means everything is in place (a failed call can be retried).
+static int ensure_snapshots(virs_ctx *c)
+{
+    if (c->snap_host) return VIRS_OK;
+    for (auto &e : c->snap_ev)
+        if (!e) HIPC(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (auto &e : c->snap_gather)
+        if (!e) HIPC(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (!c->snap_stream) HIPC(c, hipStreamCreateWithFlags(&c->snap_stream, hipStreamNonBlocking));
+    const size_t bytes = (size_t)kSnapSlots * c->nx * c->nz * sizeof(float);
+    if (!c->snap_dev) {
+        int rc = dalloc(c, &c->snap_dev, bytes);
+        if (rc) return rc;
+    }
+    if (hipHostMalloc((void **)&c->snap_host, bytes, hipHostMallocDefault) != hipSuccess) {
+        c->snap_host = nullptr;
+        return fail(c, VIRS_ERR_HIP, "pinned snapshot buffer of %zu bytes", bytes);
+    }
+    return VIRS_OK;
+}
+
+int virs_plane_request(virs_ctx *c, int64_t layer, int32_t *ticket)
+{
+    if (!c || !ticket) return VIRS_ERR_ARG;
+    if (layer < 0 || layer >= c->ny)
+        return fail(c, VIRS_ERR_ARG, "layer %lld out of range (Ny = %lld)", (long long)layer, c->ny);
+    HIPC(c, hipSetDevice(c->d.device));
+    int rc = ensure_snapshots(c);
+    if (rc) return rc;
+    const int slot = c->snap_next % kSnapSlots;
+    if (c->snap_ticket[slot] >= 0)
+        return fail(c, VIRS_ERR_ARG, "%d snapshots outstanding: fetch ticket %d first", kSnapSlots,
+                    c->snap_ticket[slot]);
+    const size_t plane = (size_t)c->nx * c->nz;
+    float *dst = c->snap_dev + slot * plane;
+    const void *src = c->f[c->ip];
+    const dim3 grid((unsigned)((c->nx + 255) / 256), (unsigned)std::min<long long>(c->nz, 65535));
+    const int nx = (int)c->nx, ny = (int)c->ny, nz = (int)c->nz, y = (int)layer;
+    switch (c->d.precision) {
+    case VIRS_FP16:
+        hipLaunchKernelGGL(k_plane_y<f16>, grid, dim3(256), 0, c->stream, (const f16 *)src, c->ctl, c->ip, dst, nx,
+                           ny, nz, y);
+        break;
+    case VIRS_BF16:
+        hipLaunchKernelGGL(k_plane_y<bf16>, grid, dim3(256), 0, c->stream, (const bf16 *)src, c->ctl, c->ip, dst,
+                           nx, ny, nz, y);
+        break;
+    case VIRS_FP64:
+        hipLaunchKernelGGL(k_plane_y<double>, grid, dim3(256), 0, c->stream, (const double *)src, c->ctl, c->ip,
+                           dst, nx, ny, nz, y);
+        break;
+    default:
+        hipLaunchKernelGGL(k_plane_y<float>, grid, dim3(256), 0, c->stream, (const float *)src, c->ctl, c->ip, dst,
+                           nx, ny, nz, y);
+        break;
+    }
+    HIPC(c, hipGetLastError());
+    // the copy waits for the gather only; later steps on the main stream do
+    // not touch the gathered plane, so they run while it copies
+    HIPC(c, hipEventRecord(c->snap_gather[slot], c->stream));
+    HIPC(c, hipStreamWaitEvent(c->snap_stream, c->snap_gather[slot], 0));
+    HIPC(c, hipMemcpyAsync(c->snap_host + slot * plane, dst, plane * sizeof(float), hipMemcpyDeviceToHost,
+                           c->snap_stream));
+    HIPC(c, hipEventRecord(c->snap_ev[slot], c->snap_stream));
+    c->snap_ticket[slot] = c->snap_next;
+    *ticket = c->snap_next;
+    c->snap_next = (c->snap_next + 1) & 0x7fffffff; // 2^31 is a multiple of kSnapSlots
+    return VIRS_OK;
+}
+
+int virs_plane_fetch(virs_ctx *c, int32_t ticket, float *out)
+{
+    if (!c || !out || ticket < 0) return VIRS_ERR_ARG;
+    const int slot = ticket % kSnapSlots;
+    if (!c->snap_host || c->snap_ticket[slot] != ticket)
+        return fail(c, VIRS_ERR_ARG, "no outstanding snapshot with ticket %d", ticket);
+    HIPC(c, hipSetDevice(c->d.device));
+    HIPC(c, hipEventSynchronize(c->snap_ev[slot]));
+    const size_t plane = (size_t)c->nx * c->nz;
+    std::memcpy(out, c->snap_host + slot * plane, plane * sizeof(float));
+    c->snap_ticket[slot] = -1;
+    return VIRS_OK;
+}
+
 int virs_memory_bytes(const virs_ctx *c, int64_t *bytes)
 {
     if (!c || !bytes) return VIRS_ERR_ARG;
@@ -1138,6 +1250,7 @@ int virs_memory_bytes(const virs_ctx *c, int64_t *bytes)
     b += 4 * (long long)compute_bytes(c) * c->trace_slots * c->nb;
     for (auto p : c->af)
         if (p) b += (long long)c->esz * c->N;
+    if (c->snap_host) b += (long long)kSnapSlots * c->nx * c->nz * (long long)sizeof(float);
     *bytes = b;
     return VIRS_OK;
 }

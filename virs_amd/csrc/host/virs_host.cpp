@@ -20,6 +20,8 @@
 #include <unordered_map>
 #include <vector>
 
+#include <zlib.h>
+
 struct virs_mesh {
     std::vector<float> xyz;      // [ntri][3][3]
     std::vector<int32_t> mat;    // [ntri]
@@ -603,3 +605,96 @@ int virs_ir_audio(const float *ir, int64_t steps, float h, int32_t rate, float *
 }
 
 } // extern "C"
+
+// ---------------------------------------------------------------------------
+// Layer images (VectorSpace::layerToImage, vectorspace.cpp:77-148).
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// std::clamp(v, 0, 255) followed by the conversion to unsigned char
+// (truncation); a NaN, which the reference converts with undefined
+// behaviour, gives 0.
+uint8_t layer_channel(float v)
+{
+    if (!(v > 0.f)) return 0;
+    if (v > 255.f) return 255;
+    return (uint8_t)v;
+}
+
+void put_be32(std::vector<unsigned char> &b, uint32_t v)
+{
+    b.push_back((unsigned char)(v >> 24));
+    b.push_back((unsigned char)(v >> 16));
+    b.push_back((unsigned char)(v >> 8));
+    b.push_back((unsigned char)v);
+}
+
+// One PNG chunk: length, type, data, CRC-32 over type and data.
+void png_chunk(std::vector<unsigned char> &out, const char *type, const unsigned char *data, size_t n)
+{
+    put_be32(out, (uint32_t)n);
+    const size_t at = out.size();
+    out.insert(out.end(), type, type + 4);
+    if (n) out.insert(out.end(), data, data + n);
+    put_be32(out, (uint32_t)crc32(0L, out.data() + at, (uInt)(n + 4)));
+}
+
+} // namespace
+
+int virs_layer_rgb(const float *plane, const uint8_t *flags, int64_t nx, int64_t ny, int64_t nz, int64_t layer,
+                   uint8_t *rgb, int32_t *width, int32_t *height)
+{
+    if (nx <= 0 || ny <= 0 || nz <= 0 || nx > (1 << 24) || nz > (1 << 24)) return -1;
+    if (layer < 0 || layer >= ny) return -2;
+    const int64_t ow = nx + (nx & 1), oh = nz + (nz & 1);
+    if (width) *width = (int32_t)ow;
+    if (height) *height = (int32_t)oh;
+    if (!rgb) return 0;
+    if (!plane || !flags) return -1;
+    std::memset(rgb, 0, (size_t)(ow * oh * 3));
+    const float scale = 1500.f;
+    for (int64_t k = 0; k < nz; ++k) {
+        const uint8_t *frow = flags + (k * ny + layer) * nx;
+        const float *prow = plane + k * nx;
+        for (int64_t i = 0; i < nx; ++i) {
+            const float p = prow[i];
+            // rows are laid out with the unpadded width, as the reference
+            // writes them, inside the image of the padded size
+            uint8_t *px = rgb + (k * nx + i) * 3;
+            px[1] = frow[i] == 1 ? 100 : frow[i] == 2 ? 255 : 0;
+            if (p >= 0.0f)
+                px[0] = layer_channel(p * scale);
+            else
+                px[2] = layer_channel(-p * scale);
+        }
+    }
+    return 0;
+}
+
+int virs_png_write_rgb(const char *path, const uint8_t *rgb, int32_t width, int32_t height)
+{
+    if (!path || !rgb || width <= 0 || height <= 0 || (int64_t)width * height > ((int64_t)1 << 28)) return -1;
+    const size_t row = (size_t)width * 3;
+    std::vector<unsigned char> raw((row + 1) * (size_t)height);
+    for (int32_t y = 0; y < height; ++y) {
+        raw[(row + 1) * y] = 0; // filter: none
+        std::memcpy(&raw[(row + 1) * y + 1], rgb + row * y, row);
+    }
+    uLongf zn = compressBound((uLong)raw.size());
+    std::vector<unsigned char> z(zn);
+    if (compress2(z.data(), &zn, raw.data(), (uLong)raw.size(), Z_DEFAULT_COMPRESSION) != Z_OK) return -4;
+    std::vector<unsigned char> out = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1a, '\n'};
+    std::vector<unsigned char> ihdr;
+    put_be32(ihdr, (uint32_t)width);
+    put_be32(ihdr, (uint32_t)height);
+    const unsigned char rest[5] = {8, 2, 0, 0, 0}; // 8-bit RGB, deflate, adaptive filtering, no interlace
+    ihdr.insert(ihdr.end(), rest, rest + 5);
+    png_chunk(out, "IHDR", ihdr.data(), ihdr.size());
+    png_chunk(out, "IDAT", z.data(), zn);
+    png_chunk(out, "IEND", nullptr, 0);
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return -2;
+    const bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+    return (std::fclose(f) == 0 && ok) ? 0 : -3;
+}

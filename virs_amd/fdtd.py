@@ -194,6 +194,19 @@ class FDTD:
         nx, ny, nz = self.dims
         return out.reshape(nz, ny, nx)
 
+    def plane_request(self, layer):
+        """Queues a snapshot of the XZ plane y = layer of p^n; returns its ticket."""
+        t = C.c_int32(-1)
+        self._chk(self.L.virs_plane_request(self._ctx, int(layer), C.byref(t)), "plane request")
+        return t.value
+
+    def plane_fetch(self, ticket):
+        """The snapshot of `ticket` as float32 [nz, nx] (waits for it alone)."""
+        nx, ny, nz = self.dims
+        out = np.zeros((nz, nx), np.float32)
+        self._chk(self.L.virs_plane_fetch(self._ctx, int(ticket), out.ctypes.data_as(_lib.c_fp)), "plane fetch")
+        return out
+
     def write_field(self, which, values):
         v = np.ascontiguousarray(np.asarray(values, np.float64).reshape(-1))
         self._chk(self.L.virs_write_field_f64(self._ctx, int(which), v.ctypes.data_as(_lib.c_dp)), "write field")
