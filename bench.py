@@ -190,7 +190,7 @@ def adjoint_cost(steps=4096, ncp=64):
             "fp32": out["fp32"], "fp64": out["fp64"], "rel_err_fp32_vs_fp64": float(f"{err:.3e}")}
 
 
-def large_room(steps=1000, warmup=20, adj_steps=40000, ncp=200):
+def large_room(steps=1000, warmup=20, adj_steps=40000, ncp=200, adjoint=True):
     """The 1536x1024x768 room (1.21 G cells, shoebox generator, 5 materials,
     8 receivers) with FP16 storage: Gvoxel-updates/s, stencil roofline and
     device footprint; then the FP32 adjoint on a 512^3 corner crop of it with
@@ -220,6 +220,8 @@ def large_room(steps=1000, warmup=20, adj_steps=40000, ncp=200):
                 "ir_finite": bool(np.isfinite(f.receivers()).all())})
     f.close()
     print(f"# large room {out['value']} Gvox/s, {out['device_bytes'] / 1e9:.2f} GB", file=sys.stderr, flush=True)
+    if not adjoint:
+        return out
     # FP32 adjoint on the 512^3 corner crop
     flags, bidx, bmat = rooms.crop(prep, (512, 512, 512))
     rng = np.random.default_rng(5)
@@ -345,7 +347,8 @@ def main():
         sys.exit(2)
 
     if args.large:
-        print(json.dumps(large_room(args.steps, args.warmup, args.adj_steps, args.adj_checkpoints)), flush=True)
+        print(json.dumps(large_room(args.steps, args.warmup, args.adj_steps, args.adj_checkpoints,
+                                    not args.no_adjoint)), flush=True)
         return
 
     from virs_amd import rooms
