@@ -6,8 +6,9 @@ One step = one leapfrog update of every cell of the 512x384x256 grid
 (50.3 M cells) plus the boundary-biquad list, the Gaussian-derivative source
 and 8 receivers, with fields already resident in HBM.  Prints one JSON line.
 
-roofline: the stencil launch's algorithmic bytes N*(3s+1) (read p^n and
-p^{n-1}, write p^{n+1}, read one code byte per cell) divided by its mean
+roofline: the stencil launch's algorithmic bytes N*(3s+c) (read p^n and
+p^{n-1}, write p^{n+1}, read the cell code: c = 1 byte per cell, half a
+byte for 16-bit storage) divided by its mean
 launch time, taken with HIP events around the two launches of every 31st
 step of the timed region (the steps between are replayed as 30-step HIP
 graphs; --no-graph launches every step and samples about 64), on the
@@ -53,12 +54,30 @@ def host_cores():
         return os.cpu_count() or 1
 
 
+def code_bytes(s):
+    """Bytes of cell code per cell: one, or a nibble for 16-bit storage."""
+    return 0.5 if s == 2 else 1
+
+
+def dense_bytes_of(n, s):
+    """Algorithmic bytes of one stencil launch: N(3s + code)."""
+    return int(n * (3 * s + code_bytes(s)))
+
+
+def list_entry_bytes(s):
+    """Algorithmic bytes per list entry: 4 index + 1 material + 2c+c
+    filter-input ring (c = compute type: 4, or 8 for FP64 fields) + 2y+y
+    filter-output ring (y = 8, or 4 for 16-bit fields) + 2s+s pressure ring
+    + s+s p^{n+1} read and written."""
+    c = 8 if s == 8 else 4
+    y = 4 if s == 2 else 8
+    return 5 + 3 * c + 3 * y + 5 * s
+
+
 def step_bytes(n, nb, nrcv, s):
-    """Algorithmic bytes of one step: N(3s+1) + Nb(41+5s) + R s.  Per list
-    entry: 4 index + 1 material + 8+4 filter-input ring (float) + 16+8
-    filter-output ring (FP64) + 2s+s pressure ring + s+s p^{n+1} read and
-    written."""
-    return n * (3 * s + 1) + nb * (41 + 5 * s) + nrcv * s
+    """Algorithmic bytes of one step: the stencil's, Nb list entries and R
+    receiver samples of s bytes."""
+    return dense_bytes_of(n, s) + nb * list_entry_bytes(s) + nrcv * s
 
 
 def run_gpu(prep, precision, steps, warmup, dense_blocks=0, fuse=False, vw=0, graph=True):
@@ -210,7 +229,7 @@ def large_room(steps=1000, warmup=20, adj_steps=40000, ncp=200, adjoint=True):
     f.step(warmup)
     f.synchronize()
     total_ms, dense_ms, bnd_ms = f.time_steps(steps)
-    db = n * (3 * BYTES["fp16"] + 1)
+    db = dense_bytes_of(n, BYTES["fp16"])
     out.update({"steps": steps, "value": round(n * steps / (total_ms / 1e3) / 1e9, 3),
                 "unit": "Gvoxel-updates/s", "ms_per_step": round(total_ms / steps, 4),
                 "roofline": {"kernel": "k_dense", "kernel_ms": round(dense_ms, 4),
@@ -366,7 +385,7 @@ def main():
     ms = r["total_ms"] / args.steps
     value = n * args.steps / (r["total_ms"] / 1e3) / 1e9
     s = r["s"]
-    dense_bytes = n * (3 * s + 1)
+    dense_bytes = dense_bytes_of(n, s)
     achieved = dense_bytes / (r["dense_ms"] / 1e3) / 1e9
     sbytes = step_bytes(n, nb, nrcv, s)
     out = {
@@ -407,7 +426,7 @@ def main():
         rv = run_gpu(prep, prec, args.steps, args.warmup, args.dense_blocks, args.fuse,
                      args.vw16 if prec in ("fp16", "bf16") else 0, graph=not args.no_graph)
         vv = n * args.steps / (rv["total_ms"] / 1e3) / 1e9
-        db = n * (3 * rv["s"] + 1)
+        db = dense_bytes_of(n, rv["s"])
         variants[prec] = {"value": round(vv, 3), "speedup_vs_main": round(vv / value, 3),
                           "dense_gbs": round(db / (rv["dense_ms"] / 1e3) / 1e9, 1),
                           "dense_ms": round(rv["dense_ms"], 5), "boundary_kernel_ms": round(rv["bnd_ms"], 5)}

@@ -609,7 +609,9 @@ int oracle_run_s16(const oracle_desc *d, int bf16, int sr, int64_t nsteps, doubl
             double *s4 = st + 4 * e;
             const double in = (double)dif;
             const double yv = c[0] * in + c[1] * s4[0] + c[2] * s4[1] - c[3] * s4[2] - c[4] * s4[3];
-            s4[1] = s4[0]; s4[0] = in; s4[3] = s4[2]; s4[2] = yv;
+            /* the stored output is rounded to float; this step's correction
+             * uses the double value */
+            s4[1] = s4[0]; s4[0] = in; s4[3] = s4[2]; s4[2] = (double)(float)yv;
             float r = ld16(&S, R[bi]) * sRi;
             r = (float)((double)r - (double)d->kconst * (yv * 0.1 + in * 0.9) * (double)0.4f);
             R[bi] = st16(&S, r * sR, (uint64_t)bi, n, SALT_BOUNDARY);

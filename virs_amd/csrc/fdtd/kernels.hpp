@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #pragma clang fp contract(off)
 
 namespace virs {
@@ -293,35 +295,62 @@ __global__ void k_build_code(const uint8_t *__restrict__ flags, uint8_t *__restr
     }
 }
 
-// Cell code of the 16-bit formats.  In 16-bit storage every wall and border
-// cell holds zero, so a wall neighbour contributes nothing to the plain sum of
-// the six neighbours and the reference's substitution V(wall) = pc becomes a
-// smaller centre weight: lap = (sum of neighbours - kw pc) ih2 with
-// kw = 6 - (wall neighbours).  Byte: kw for cells that update, kCode16Zero
-// for walls and border cells, kCode16Hard for hard-source cells.
-constexpr uint8_t kCode16Zero = 0x40, kCode16Hard = 0x80;
+// Cell code of the 16-bit formats, half a byte per cell: cell i sits in the
+// low nibble of byte i/2 when i is even, in the high nibble when odd.  In
+// 16-bit storage every wall and border cell holds zero, so a wall neighbour
+// contributes nothing to the plain sum of the six neighbours and the
+// reference's substitution V(wall) = pc becomes a smaller centre weight:
+// lap = (sum of neighbours - kw pc) ih2 with kw = 6 - (wall neighbours).
+// Nibble: kw (0..6) for cells that update, kNib16Zero for walls and border
+// cells, kNib16Hard for hard-source cells (bit 3 marks both, bit 0 tells
+// them apart).
+constexpr uint32_t kNib16Zero = 8, kNib16Hard = 9;
 __global__ void k_build_code16(const uint8_t *__restrict__ flags, uint8_t *__restrict__ code, int nx,
                                int ny, int nz)
 {
-    const long long sxy = (long long)nx * ny, n = sxy * nz;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x) {
-        const int z = (int)(i / sxy);
-        const long long r = i - (long long)z * sxy;
-        const int y = (int)(r / nx);
-        const int x = (int)(r - (long long)y * nx);
-        uint8_t c;
-        if (x == 0 || y == 0 || z == 0 || x == nx - 1 || y == ny - 1 || z == nz - 1 || flags[i] == 1) {
-            c = kCode16Zero;
-        } else if (flags[i] == 3) {
-            c = kCode16Hard;
-        } else {
-            const int walls = (flags[i + 1] == 1) + (flags[i - 1] == 1) + (flags[i + nx] == 1) +
-                              (flags[i - nx] == 1) + (flags[i + sxy] == 1) + (flags[i - sxy] == 1);
-            c = (uint8_t)(6 - walls);
+    const long long sxy = (long long)nx * ny, n = sxy * nz, nbytes = (n + 1) / 2;
+    for (long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x; b < nbytes;
+         b += (long long)gridDim.x * blockDim.x) {
+        uint32_t pair = 0;
+        for (int h = 0; h < 2; ++h) {
+            const long long i = 2 * b + h;
+            uint32_t c = kNib16Zero; // the pad nibble past the last cell
+            if (i < n) {
+                const int z = (int)(i / sxy);
+                const long long r = i - (long long)z * sxy;
+                const int y = (int)(r / nx);
+                const int x = (int)(r - (long long)y * nx);
+                if (x == 0 || y == 0 || z == 0 || x == nx - 1 || y == ny - 1 || z == nz - 1 || flags[i] == 1) {
+                    c = kNib16Zero;
+                } else if (flags[i] == 3) {
+                    c = kNib16Hard;
+                } else {
+                    const int walls = (flags[i + 1] == 1) + (flags[i - 1] == 1) + (flags[i + nx] == 1) +
+                                      (flags[i - nx] == 1) + (flags[i + sxy] == 1) + (flags[i - sxy] == 1);
+                    c = (uint32_t)(6 - walls);
+                }
+            }
+            pair |= c << (4 * h);
         }
-        code[i] = c;
+        code[b] = (uint8_t)pair;
     }
+}
+
+__device__ inline uint32_t code16_at(const uint8_t *__restrict__ code, long long i)
+{
+    return ((uint32_t)code[i >> 1] >> ((uint32_t)(i & 1) * 4u)) & 0xFu;
+}
+
+// The nibbles of cells i .. i+V-1 (i a multiple of V), cell i + j in bits
+// 4j .. 4j+3, with one streaming load of V/2 bytes (half a byte for V = 1).
+template <int V>
+__device__ inline uint32_t load_nibbles(const uint8_t *__restrict__ code, long long i)
+{
+    const uint8_t *b = code + (i >> 1);
+    if constexpr (V == 8) return __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(b));
+    else if constexpr (V == 4) return (uint32_t)__builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(b));
+    else if constexpr (V == 2) return (uint32_t)__builtin_nontemporal_load(b);
+    else return (uint32_t)__builtin_nontemporal_load(b) >> ((uint32_t)(i & 1) * 4u);
 }
 
 // Zeroes the cells a 16-bit code marks as walls or border.
@@ -330,7 +359,7 @@ __global__ void k_zero_walls(T *__restrict__ f, const uint8_t *__restrict__ code
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (long long)gridDim.x * blockDim.x)
-        if (code[i] & kCode16Zero) f[i] = Store<T>::st(0.f);
+        if (code16_at(code, i) == kNib16Zero) f[i] = Store<T>::st(0.f);
 }
 
 // All-ones if bit `bit` of c is set, else zero (one signed bit-field extract).
@@ -374,10 +403,10 @@ struct BoundaryState {
     // step n-1; step n overwrites slot n&1, so a step writes one value per
     // ring instead of shifting the state.  Forward: xs = filter inputs d in
     // the field's compute type C (d is a difference of two C values, so C
-    // holds it exactly), ys = outputs y in FP64.  Reverse: xs = dbar,
-    // ys = ybar, both FP64.
+    // holds it exactly), ys = outputs y in FP64, or FP32 for 16-bit storage
+    // (YsOf).  Reverse: xs = dbar, ys = ybar, both FP64.
     void *xs[2];
-    double *ys[2];
+    void *ys[2];
     const double *coeffs;      // [nmat][5] b0 b1 b2 a1 a2
     int nmat;
     float kconst;              // (c*dt)/h*0.95f
@@ -418,22 +447,39 @@ struct DenseParams {
 
 // Filter taps of entry e before step n: x1 = d^{n-1}, x2 = d^{n-2},
 // y1 = y^{n-1}, y2 = y^{n-2}.
-template <typename C>
+// Element type of the filter-output ring for field storage T: FP64, or
+// FP32 for the 16-bit formats, whose storage rounding is far coarser than
+// FP32's (the reference keeps all of its filter state in float).  The
+// output is still computed in double; only the stored copy is rounded.
+template <typename T>
+struct YsOf {
+    using type = double;
+};
+template <>
+struct YsOf<f16> {
+    using type = float;
+};
+template <>
+struct YsOf<bf16> {
+    using type = float;
+};
+
+template <typename C, typename Y>
 __device__ inline void load_taps(const BoundaryState &bs, long long e, long long n, double &x1, double &x2,
                                  double &y1, double &y2)
 {
     const int s = (int)(n & 1);
     x1 = (double)((const C *)bs.xs[s ^ 1])[e];
     x2 = (double)((const C *)bs.xs[s])[e];
-    y1 = bs.ys[s ^ 1][e];
-    y2 = bs.ys[s][e];
+    y1 = (double)((const Y *)bs.ys[s ^ 1])[e];
+    y2 = (double)((const Y *)bs.ys[s])[e];
 }
 
 // DF1 biquad step and boundary correction for list entry e (simulator_gpu.cu:126-148):
-//   y = b0 d + b1 x1 + b2 x2 - a1 y1 - a2 y2   (FP64 state)
+//   y = b0 d + b1 x1 + b2 x2 - a1 y1 - a2 y2   (in double; stored as Y)
 //   R = R - K (0.1 y + 0.9 d) 0.4f             (evaluated in double)
 // x1..y2 is the entry's state as loaded before the call.
-template <typename C>
+template <typename C, typename Y>
 __device__ inline C boundary_update_pre(const BoundaryState &bs, long long e, long long n, C dif, C r,
                                         double x1, double x2, double y1, double y2)
 {
@@ -442,7 +488,7 @@ __device__ inline C boundary_update_pre(const BoundaryState &bs, long long e, lo
     const double in = (double)dif;
     const double yv = cf[0] * in + cf[1] * x1 + cf[2] * x2 - cf[3] * y1 - cf[4] * y2;
     ((C *)bs.xs[n & 1])[e] = dif;
-    bs.ys[n & 1][e] = yv;
+    ((Y *)bs.ys[n & 1])[e] = (Y)yv;
     if (bs.trace_d) {
         const long long slot = n - bs.trace_base + 2;
         if (slot >= 0 && slot < bs.trace_slots) {
@@ -453,12 +499,12 @@ __device__ inline C boundary_update_pre(const BoundaryState &bs, long long e, lo
     return (C)((double)r - (double)bs.kconst * (yv * 0.1 + in * 0.9) * (double)0.4f);
 }
 
-template <typename C>
+template <typename C, typename Y>
 __device__ inline C boundary_update(const BoundaryState &bs, long long e, long long n, C dif, C r)
 {
     double x1, x2, y1, y2;
-    load_taps<C>(bs, e, n, x1, x2, y1, y2);
-    return boundary_update_pre<C>(bs, e, n, dif, r, x1, x2, y1, y2);
+    load_taps<C, Y>(bs, e, n, x1, x2, y1, y2);
+    return boundary_update_pre<C, Y>(bs, e, n, dif, r, x1, x2, y1, y2);
 }
 
 // Leapfrog 7-point update, simulator_gpu.cu:38-103 with a graded sponge:
@@ -572,9 +618,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
         wend = wbeg + per + ((long long)logical < rem ? 1 : 0);
     }
 
+    // cell codes of one plane: V bytes, or V nibbles in one word for 16-bit
+    using CD = typename std::conditional<S::kScaled, uint32_t, VB>::type;
     struct Stage {
         VT pp, q, yn, ys; // P(z+1), Q(z), P(z,y+1), P(z,y-1) as stored
-        VB cd;            // cell codes of plane z
+        CD cd;            // cell codes of plane z
         T tl, tr;         // P(z) left of lane 0 / right of lane 63
     };
 
@@ -608,10 +656,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
             const char *Pz = reinterpret_cast<const char *>(P + (long long)z * sxy);
             const char *Pzp = reinterpret_cast<const char *>(P + (long long)z * sxy + (z + 1 < nz ? sxy : 0));
             const char *Qz = reinterpret_cast<const char *>(Q + (long long)z * sxy);
-            const uint8_t *Cz = p.code + (long long)z * sxy;
             st.pp = vload<VT, false>(reinterpret_cast<const VT *>(Pzp + oc));
             st.q = vload<VT, true>(reinterpret_cast<const VT *>(Qz + oc));
-            st.cd = vload<VB, true>(reinterpret_cast<const VB *>(Cz + ob));
+            if constexpr (S::kScaled)
+                st.cd = load_nibbles<V>(p.code, (long long)z * sxy + cell0);
+            else
+                st.cd = vload<VB, true>(reinterpret_cast<const VB *>(p.code + (long long)z * sxy + ob));
             st.yn = vload<VT, false>(reinterpret_cast<const VT *>(Pz + on));
             st.ys = vload<VT, false>(reinterpret_cast<const VT *>(Pz + os));
             if (need_l) st.tl = reinterpret_cast<const T *>(Pz + oc)[-1];
@@ -645,7 +695,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                 qv[j] = S::ld(cur.q.v[j]);
                 yn[j] = S::ld(cur.yn.v[j]);
                 ys[j] = S::ld(cur.ys.v[j]);
-                cd[j] = (uint32_t)cur.cd.v[j];
+                if constexpr (S::kScaled)
+                    cd[j] = (cur.cd >> (4 * j)) & 0xFu;
+                else
+                    cd[j] = (uint32_t)cur.cd.v[j];
             }
             if constexpr (S::kScaled) {
                 // Q to P units; consecutive steps mostly share an exponent,
@@ -671,7 +724,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                     sum = sum + ys[j];
                     sum = sum + PP[j];
                     sum = sum + PM[j];
-                    const C lapc = (sum - (C)c * pcv) * c2ih2; // (C)c = kw for the cells that update
+                    const C lapc = (sum - (C)(c & 7u) * pcv) * c2ih2; // c & 7 = kw for the cells that update
                     C a = (C)2, b = (C)1, g = (C)1;
                     if constexpr (SPONGE) {
                         const int d = edge_dist(x0 + j, y, z, nx, ny, nz);
@@ -682,7 +735,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                         }
                     }
                     const C upd = (a * pcv - b * qv[j]) + g * lapc;
-                    o[j] = bsel(bit_mask(c, 6), (C)0, bsel(bit_mask(c, 7), hard_u, upd));
+                    o[j] = bsel(bit_mask(c, 3), bsel(bit_mask(c, 0), hard_u, (C)0), upd);
                     continue;
                 }
                 const C xp = bsel(bit_mask(c, 0), pcv, j + 1 < V ? PC[(j + 1) % V] : right);
@@ -831,7 +884,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                     pv[u] = S::ld(P[bv[u]]);
                     qv2[u] = S::ld(Q[bv[u]]);
                     rv[u] = S::ld(R[bv[u]]);
-                    load_taps<C>(p.bs, ev[u], n, x1[u], x2[u], y1[u], y2[u]);
+                    load_taps<C, typename YsOf<T>::type>(p.bs, ev[u], n, x1[u], x2[u], y1[u], y2[u]);
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -843,7 +896,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                         qq = qq * kphys * kq;
                         r = r * kphys / kout;
                     }
-                    r = boundary_update_pre<C>(p.bs, ev[u], n, pp - qq, r, x1[u], x2[u], y1[u], y2[u]);
+                    r = boundary_update_pre<C, typename YsOf<T>::type>(p.bs, ev[u], n, pp - qq, r, x1[u], x2[u], y1[u], y2[u]);
                     if constexpr (S::kScaled) {
                         R[bv[u]] = store_round<T>(r * (kout / kphys), SR, (unsigned long long)bv[u], n,
                                                   kSaltBoundary);
@@ -974,7 +1027,7 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
             qv[u] = S::ld(Q[bv[u]]);
         }
         rv[u] = S::ld(R[bv[u]]);
-        load_taps<C>(p.bs, ev[u], n, x1[u], x2[u], y1[u], y2[u]);
+        load_taps<C, typename YsOf<T>::type>(p.bs, ev[u], n, x1[u], x2[u], y1[u], y2[u]);
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -982,7 +1035,7 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
         const long long e = ev[u], b = bv[u];
         C pp = pv[u], qq = qv[u], r = rv[u];
         if constexpr (S::kScaled) pp = pp * sP, qq = qq * sQ, r = r * sRi;
-        r = boundary_update_pre<C>(p.bs, e, n, pp - qq, r, x1[u], x2[u], y1[u], y2[u]);
+        r = boundary_update_pre<C, typename YsOf<T>::type>(p.bs, e, n, pp - qq, r, x1[u], x2[u], y1[u], y2[u]);
         T t = store_round<T>(r * sR, p.sr, (unsigned long long)b, n, kSaltBoundary);
         // a soft source on this cell is added to the stored corrected value
         for (int k = 0; k < p.nsoft; ++k)
@@ -1083,7 +1136,7 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
         }
         const int sp = (int)(n & 1);
         const double *dbr0 = (const double *)bs.xs[sp ^ 1], *dbr1 = (const double *)bs.xs[sp];
-        const double yb1 = bs.ys[sp ^ 1][e], yb2 = bs.ys[sp][e], db1 = dbr0[e], db2 = dbr1[e];
+        const double yb1 = ((const double *)bs.ys[sp ^ 1])[e], yb2 = ((const double *)bs.ys[sp])[e], db1 = dbr0[e], db2 = dbr1[e];
         rb = rb + db1 - db2;
         if (in_g)
             for (int k = 0; k < p.nrcv; ++k)
@@ -1103,7 +1156,7 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
         gv[2] = yb * x2;
         gv[3] = -yb * y1;
         gv[4] = -yb * y2;
-        bs.ys[sp][e] = yb;
+        ((double *)bs.ys[sp])[e] = yb;
         ((double *)bs.xs[sp])[e] = db;
         R[b] = (C)(spg ? rb * g : rb);
     }

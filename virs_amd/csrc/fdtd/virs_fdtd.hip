@@ -201,8 +201,8 @@ BoundaryState boundary_state(const virs_ctx *c, void *const *st)
     b.bmat = c->bmat;
     b.xs[0] = st[0];
     b.xs[1] = st[1];
-    b.ys[0] = (double *)st[2];
-    b.ys[1] = (double *)st[3];
+    b.ys[0] = st[2];
+    b.ys[1] = st[3];
     b.coeffs = c->coeffs;
     b.nmat = c->nmat;
     b.kconst = c->kconst;
@@ -274,11 +274,19 @@ TailParams tail_params(const virs_ctx *c, const void *P, const void *Q, void *R,
 
 int tail_grid(long long nb) { return (int)std::max<long long>(1, (nb + kBlock - 1) / kBlock); }
 
-// Stencil grid.  capacity = resident blocks of the whole GPU (occupancy x
-// CUs).  Tile mode (one block per chunk x z-range) is used when whole
-// z-ranges of every chunk fill at least 85% of the capacity in one round;
-// otherwise the (chunk, z) planes are split into `capacity` equal ranges.
-// A descriptor value dense_blocks > 0 forces the split mode with that grid.
+// Stencil grid.  capacity = resident blocks of the whole GPU: occupancy x
+// CUs, the occupancy capped at VIRS_DENSE_OCC_CAP blocks (12 waves) per CU.
+// Fewer blocks that each march more planes beat more resident waves here:
+// on the 512x384x256 room, planning for 2 / 3 / 4 / 5 / 6 blocks per CU
+// gave FP32 steps of 363 / 399 / 377 / 373 / 390 Gvox/s and FP16 steps of
+// 557 / 648 / 626 / 626 Gvox/s.  Tile mode (one block per chunk x z-range)
+// is used when whole z-ranges of every chunk fill at least 85% of the
+// capacity in one round; otherwise the (chunk, z) planes are split into
+// `capacity` equal ranges.  A descriptor value dense_blocks > 0 forces the
+// split mode with that grid.
+#ifndef VIRS_DENSE_OCC_CAP
+#define VIRS_DENSE_OCC_CAP 3 // blocks per CU the stencil grid plans for at most (0 = no bound)
+#endif
 template <typename K>
 int dense_grid(const virs_ctx *c, K kernel, int &zranges)
 {
@@ -288,6 +296,7 @@ int dense_grid(const virs_ctx *c, K kernel, int &zranges)
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kBlock, 0) != hipSuccess || occ <= 0)
         occ = 4;
+    if (VIRS_DENSE_OCC_CAP > 0) occ = std::min(occ, VIRS_DENSE_OCC_CAP);
     const long long cap = (long long)occ * std::max(c->cus, 1);
     const long long zr = std::min<long long>(cap / std::max(c->nchunks_plane, 1), c->nz);
     if (zr >= 1 && zr * c->nchunks_plane * 100 >= cap * 85) {
@@ -489,11 +498,17 @@ int check_ready(virs_ctx *c)
 // bytes of the compute type C of a field (float, or double for FP64): the
 // element of the filter-input rings and of the adjoint trace
 size_t compute_bytes(const virs_ctx *c) { return c->d.precision == VIRS_FP64 ? 8 : 4; }
+bool is16(const virs_ctx *c) { return c->d.precision == VIRS_FP16 || c->d.precision == VIRS_BF16; }
+// cell codes: a byte per cell, half a byte in the 16-bit formats
+size_t code_bytes(const virs_ctx *c) { return is16(c) ? (size_t)(c->N + 1) / 2 : (size_t)c->N; }
 
 // bytes of filter-state ring k (0, 1: inputs in the compute type; 2, 3: FP64 outputs)
+// filter-state rings: inputs in the compute type, outputs in FP64 (FP32
+// for 16-bit storage, YsOf)
 size_t state_bytes(const virs_ctx *c, int k)
 {
-    return (k < 2 ? compute_bytes(c) : sizeof(double)) * (size_t)std::max<long long>(1, c->nb);
+    const size_t ys = is16(c) ? sizeof(float) : sizeof(double);
+    return (k < 2 ? compute_bytes(c) : ys) * (size_t)std::max<long long>(1, c->nb);
 }
 
 size_t elem_size(int prec)
@@ -613,7 +628,7 @@ int virs_create(const virs_desc *d, virs_ctx **out)
     int rc = VIRS_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = VIRS_ERR_HIP;
     for (int k = 0; k < 3 && !rc; ++k) rc = dalloc(c, &c->f[k], c->esz * (size_t)c->N);
-    if (!rc) rc = dalloc(c, &c->code, (size_t)c->N);
+    if (!rc) rc = dalloc(c, &c->code, code_bytes(c));
     if (!rc) rc = dalloc(c, &c->ctl, sizeof(StepCtl));
     if (!rc) rc = dalloc(c, &c->ctl_adj, sizeof(StepCtl));
     if (!rc) rc = dalloc(c, &c->soft_idx, sizeof(long long) * kMaxPoints);
@@ -629,7 +644,8 @@ int virs_create(const virs_desc *d, virs_ctx **out)
         hipLaunchKernelGGL(k_ctl_es, dim3(1), dim3(64), 0, c->stream, c->ctl_adj, 0, c->es_target);
         if (hipGetLastError() != hipSuccess) rc = VIRS_ERR_HIP;
     }
-    if (!rc && hipMemsetAsync(c->code, kTypeZero << 6, (size_t)c->N, c->stream) != hipSuccess) rc = VIRS_ERR_HIP;
+    const int zero_code = is16(c) ? (int)(kNib16Zero | kNib16Zero << 4) : (int)(kTypeZero << 6);
+    if (!rc && hipMemsetAsync(c->code, zero_code, code_bytes(c), c->stream) != hipSuccess) rc = VIRS_ERR_HIP;
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = VIRS_ERR_HIP;
     if (rc) {
         virs_destroy(c);
@@ -1241,7 +1257,7 @@ int virs_plane_fetch(virs_ctx *c, int32_t ticket, float *out)
 int virs_memory_bytes(const virs_ctx *c, int64_t *bytes)
 {
     if (!c || !bytes) return VIRS_ERR_ARG;
-    long long b = 3 * (long long)c->esz * c->N + c->N; // fields + cell codes
+    long long b = 3 * (long long)c->esz * c->N + (long long)code_bytes(c); // fields + cell codes
     if (c->normals) b += c->N;
     b += c->nb * (4 + 2 * (long long)compute_bytes(c) + 16 + 2 * (long long)c->esz + (c->bmat ? 1 : 0));
     b += (long long)sizeof(double) * std::max(1, c->nrcv) * std::max<long long>(1, c->ir_cap);
