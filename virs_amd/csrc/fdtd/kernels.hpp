@@ -341,16 +341,21 @@ __device__ inline uint32_t code16_at(const uint8_t *__restrict__ code, long long
     return ((uint32_t)code[i >> 1] >> ((uint32_t)(i & 1) * 4u)) & 0xFu;
 }
 
-// The nibbles of cells i .. i+V-1 (i a multiple of V), cell i + j in bits
-// 4j .. 4j+3, with one streaming load of V/2 bytes (half a byte for V = 1).
-template <int V>
-__device__ inline uint32_t load_nibbles(const uint8_t *__restrict__ code, long long i)
+// The nibble of cell i in bits 0..3 (the rest garbage), one streaming byte load.
+__device__ inline uint32_t load_nibble1(const uint8_t *__restrict__ code, long long i)
 {
-    const uint8_t *b = code + (i >> 1);
+    return (uint32_t)__builtin_nontemporal_load(code + (i >> 1)) >> ((uint32_t)(i & 1) * 4u);
+}
+
+// The nibbles of V >= 2 cells that start at byte `off` of `code` (an even
+// cell), cell j in bits 4j .. 4j+3, with one streaming load of V/2 bytes.
+template <int V>
+__device__ inline uint32_t load_nibbles(const uint8_t *__restrict__ code, uint32_t off)
+{
+    const uint8_t *b = code + off;
     if constexpr (V == 8) return __builtin_nontemporal_load(reinterpret_cast<const uint32_t *>(b));
     else if constexpr (V == 4) return (uint32_t)__builtin_nontemporal_load(reinterpret_cast<const unsigned short *>(b));
-    else if constexpr (V == 2) return (uint32_t)__builtin_nontemporal_load(b);
-    else return (uint32_t)__builtin_nontemporal_load(b) >> ((uint32_t)(i & 1) * 4u);
+    else return (uint32_t)__builtin_nontemporal_load(b);
 }
 
 // Zeroes the cells a 16-bit code marks as walls or border.
@@ -651,21 +656,36 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
         const uint32_t on = oc + (uint32_t)dyn * (uint32_t)sizeof(T);
         const uint32_t os = oc - (uint32_t)dys * (uint32_t)sizeof(T);
         const uint32_t ob = (uint32_t)cell0;
+        // The empty asm makes an offset opaque where it is used, so its
+        // zero-extension stays next to the access and the access takes the
+        // scalar-base + 32-bit vector-offset form; otherwise the extension is
+        // hoisted out of the march and every access builds a 64-bit address.
+        auto lane_off = [](uint32_t o) {
+            asm volatile("" : "+v"(o));
+            return o;
+        };
+        // stochastic-rounding hash: cell0's part of the pair index times K1
+        const uint32_t c0k1 = ((uint32_t)cell0 & ~1u) * kHashK1;
 
         auto issue = [&](int z, Stage &st) {
             const char *Pz = reinterpret_cast<const char *>(P + (long long)z * sxy);
             const char *Pzp = reinterpret_cast<const char *>(P + (long long)z * sxy + (z + 1 < nz ? sxy : 0));
             const char *Qz = reinterpret_cast<const char *>(Q + (long long)z * sxy);
-            st.pp = vload<VT, false>(reinterpret_cast<const VT *>(Pzp + oc));
-            st.q = vload<VT, true>(reinterpret_cast<const VT *>(Qz + oc));
-            if constexpr (S::kScaled)
-                st.cd = load_nibbles<V>(p.code, (long long)z * sxy + cell0);
-            else
-                st.cd = vload<VB, true>(reinterpret_cast<const VB *>(p.code + (long long)z * sxy + ob));
-            st.yn = vload<VT, false>(reinterpret_cast<const VT *>(Pz + on));
-            st.ys = vload<VT, false>(reinterpret_cast<const VT *>(Pz + os));
-            if (need_l) st.tl = reinterpret_cast<const T *>(Pz + oc)[-1];
-            if (need_r) st.tr = reinterpret_cast<const T *>(Pz + oc)[V];
+            const uint32_t xc = lane_off(oc);
+            st.pp = vload<VT, false>(reinterpret_cast<const VT *>(Pzp + xc));
+            st.q = vload<VT, true>(reinterpret_cast<const VT *>(Qz + xc));
+            if constexpr (S::kScaled) {
+                if constexpr (V == 1)
+                    st.cd = load_nibble1(p.code, (long long)z * sxy + cell0);
+                else // V divides Nx, so planes and vectors start on even cells
+                    st.cd = load_nibbles<V>(p.code + (((long long)z * sxy) >> 1), lane_off(ob >> 1));
+            } else {
+                st.cd = vload<VB, true>(reinterpret_cast<const VB *>(p.code + (long long)z * sxy + lane_off(ob)));
+            }
+            st.yn = vload<VT, false>(reinterpret_cast<const VT *>(Pz + lane_off(on)));
+            st.ys = vload<VT, false>(reinterpret_cast<const VT *>(Pz + lane_off(os)));
+            if (need_l) st.tl = reinterpret_cast<const T *>(Pz + xc)[-1];
+            if (need_r) st.tr = reinterpret_cast<const T *>(Pz + xc)[V];
         };
 
         C B0[V], B1[V], B2[V];
@@ -684,7 +704,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
         // One plane: PM/PC hold P(z-1)/P(z); PP receives P(z+1) from cur;
         // nxt receives the loads of plane z+1 before plane z is computed.
         auto body = [&](const int z, C(&PM)[V], C(&PC)[V], C(&PP)[V], Stage &cur, Stage &nxt) {
-            const long long base = (long long)z * sxy + cell0;
             C qv[V], yn[V], ys[V];
             uint32_t cd[V];
 #pragma unroll
@@ -774,14 +793,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
 #pragma unroll
                     for (int j = 0; j < V; ++j) o[j] = o[j] * kout;
                 if constexpr (SR) {
-                    const uint32_t hk = hash_hk((unsigned long long)(base & ~1ll), n, kSaltDense);
-                    const uint32_t lo = (uint32_t)(base & ~1ll) * kHashK1;
+                    // the hash of mix16 for cell i = z*sxy + cell0 in 32-bit
+                    // pieces: hk takes the high word of i (the plane's plus a
+                    // carry), lo the pair's low word times K1
+                    const unsigned long long zs = (unsigned long long)z * (unsigned long long)sxy;
+                    const uint32_t zlo = (uint32_t)zs, ilo = zlo + (uint32_t)cell0;
+                    const uint32_t hk0 = hash_hk(zs, n, kSaltDense);
+                    const uint32_t hk = ilo < zlo ? hk0 + kHashK3 : hk0;
                     if constexpr (V == 1) {
-                        const uint32_t h = hash_pair(lo, hk);
-                        const float q0 = sr_bits<T>(o[0], (base & 1) ? (h >> 16) : (h & 0xffffu));
+                        const uint32_t h = hash_pair((ilo & ~1u) * kHashK1, hk);
+                        const float q0 = sr_bits<T>(o[0], (ilo & 1u) ? (h >> 16) : (h & 0xffffu));
                         T dummy;
                         cvt2_truncated<T>(q0, 0.f, out.v[0], dummy);
                     } else {
+                        // z*sxy and cell0 are even here
+                        const uint32_t lo = zlo * kHashK1 + c0k1;
 #pragma unroll
                         for (int j = 0; j < V; j += 2) {
                             const uint32_t h = hash_pair(lo + (uint32_t)j * kHashK1, hk);
@@ -798,7 +824,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
 #pragma unroll
                 for (int j = 0; j < V; ++j) out.v[j] = S::st(o[j]);
             }
-            if (active) vstore<VT, true>(reinterpret_cast<VT *>(reinterpret_cast<char *>(R + (long long)z * sxy) + oc), out);
+            if (active) vstore<VT, true>(reinterpret_cast<VT *>(reinterpret_cast<char *>(R + (long long)z * sxy) + lane_off(oc)), out);
         };
 
         // planes rotate with period 3, load stages with period 2
@@ -974,10 +1000,16 @@ __device__ inline void sponge_g(const TailParams &p, long long idx, C &g)
 // all their loads before any store: the entries are distinct cells, so no
 // store of the batch feeds a load of it.  p^n and p^{n-1} come from the
 // coalesced ring hv instead of two scattered gathers.
+#ifndef VIRS_TAIL_BATCH16
+#define VIRS_TAIL_BATCH16 1 // entries per thread with 16-bit fields (measured)
+#endif
+#ifndef VIRS_TAIL_BATCH32
+#define VIRS_TAIL_BATCH32 1 // entries per thread with FP32/FP64 fields (measured)
+#endif
 template <typename T>
 constexpr int tail_batch()
 {
-    return Store<T>::kScaled ? 2 : 1; // measured: fewer 16-bit blocks (and |p| atomics) pay
+    return Store<T>::kScaled ? VIRS_TAIL_BATCH16 : VIRS_TAIL_BATCH32;
 }
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
