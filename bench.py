@@ -8,11 +8,13 @@ and 8 receivers, with fields already resident in HBM.  Prints one JSON line.
 
 roofline: the stencil launch's algorithmic bytes N*(3s+c) (read p^n and
 p^{n-1}, write p^{n+1}, read the cell code: c = 1 byte per cell, half a
-byte for 16-bit storage) divided by its mean
-launch time, taken with HIP events around the two launches of every 31st
-step of the timed region (the steps between are replayed as 30-step HIP
-graphs; --no-graph launches every step and samples about 64), on the
-library's own stream.
+byte for 16-bit storage) divided by its mean launch time, taken with HIP
+events around the two launches of every 31st step of the timed region (the
+steps between are replayed as 30-step HIP graphs; --no-graph launches every
+step and samples about 64), on the library's own stream.
+step_hbm_frac: the whole step's algorithmic bytes (stencil, list launch,
+receivers) per step time; step_hbm_frac_fp64_state_formula prices the same
+step with N(3s+1) + Nb(69+4s) + R s instead.
 cpu_baseline: the CPU oracle (the same scheme, OpenMP over z) on a 128^3
 corner crop of the same room, on this host's cores.
 """
@@ -72,6 +74,14 @@ def list_entry_bytes(s):
     c = 8 if s == 8 else 4
     y = 4 if s == 2 else 8
     return 5 + 3 * c + 3 * y + 5 * s
+
+
+def step_bytes_fp64_state(n, nb, nrcv, s):
+    """Step bytes by the fixed formula N(3s+1) + Nb(69+4s) + R s: a byte of
+    flags per cell, and per list entry an index, a material, p, p^{n-1},
+    p^{n+1} read-modify-write and four FP64 filter-state values read and
+    written (the reference's state layout, widened to FP64)."""
+    return n * (3 * s + 1) + nb * (69 + 4 * s) + nrcv * s
 
 
 def step_bytes(n, nb, nrcv, s):
@@ -413,6 +423,9 @@ def main():
                      "bytes_per_launch": dense_bytes},
         "step_algorithmic_gbs": round(sbytes / (ms / 1e3) / 1e9, 1),
         "step_hbm_frac": round(sbytes / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        # the same step priced with the fixed FP64-state formula
+        "step_hbm_frac_fp64_state_formula": round(step_bytes_fp64_state(n, nb, nrcv, s) / (ms / 1e3) / 1e9
+                                                  / HBM_PEAK_GBS, 4),
         "boundary_kernel_ms": round(r["bnd_ms"], 5),
         "device_bytes": r["mem"],
         # context creation + host->device upload of flags, normals, list and
