@@ -5,8 +5,8 @@
    oracle's restatement of the same storage rules, on small grids.
 2. Accuracy against the FP64 run of the same scheme on the 5x4x3 m shoebox
    with 8 receivers (2000 steps); the largest relative L2 error over the
-   receivers, measured: FP32 1.9e-5; FP16 4.5e-3 with stochastic rounding,
-   5.4e-3 with nearest-even rounding; BF16 3.4e-2 (stochastic).  The FP16
+   receivers, measured: FP32 1.9e-5; FP16 4.8e-3 with stochastic rounding,
+   5.0e-3 with nearest-even rounding; BF16 3.3e-2 (stochastic).  The FP16
    error is a noise floor of the 11-bit significand: the per-store rounding
    noise accumulates over the room's decay time.  The 1e-3 goal for FP16 is
    not met; the bounds below pin the measured levels.
@@ -46,6 +46,26 @@ def test_16bit_storage_bit_exact(gpu, fmt, rounding, seed):
     np.testing.assert_array_equal(f.field(1), ref["q"])
     assert f.field_exponent(0) == ref["exp"][-1]
     f.close()
+
+
+@pytest.mark.parametrize("fmt", ["fp16", "bf16"])
+@pytest.mark.parametrize("nx", [23, 26, 28, 40])
+def test_16bit_odd_widths(gpu, fmt, nx):
+    """Row widths that force 1-, 2-, 4- and 8-wide vectors in 16-bit
+    storage, so the half-byte cell codes are read half a byte, one, two or
+    four bytes at a time (with Nx = 23 and Ny = 15 every other plane starts
+    on an odd cell), bit-exact with the oracle in both roundings."""
+    p = tiny_problem(12, dims=(nx, 15, 13), steps=40)
+    for rounding in ("stochastic", "nearest"):
+        print(f"16-bit odd width {fmt} nx={nx} {rounding}", flush=True)
+        f = gpu_context(p, fmt, rounding=rounding)
+        f.step(p["steps"])
+        f.synchronize()
+        ref = oracle_problem(p).run16(p["steps"], fmt, rounding == "stochastic", want_fields=True)
+        np.testing.assert_array_equal(f.receivers(True), ref["ir"])
+        np.testing.assert_array_equal(f.field(0), ref["p"])
+        np.testing.assert_array_equal(f.field(1), ref["q"])
+        f.close()
 
 
 @pytest.fixture(scope="module")

@@ -255,7 +255,8 @@ __device__ inline int wave_max_int(int v)
     return v;
 }
 
-// Cell code, one byte per cell, built on the device from the reference flags:
+// Cell code of FP32/FP64 fields, one byte per cell, built on the device from
+// the reference flags (16-bit fields use k_build_code16's half-byte code):
 //   bits 0-5  neighbour is a wall (flag 1): +x 1, -x 2, +y 4, -y 8, +z 16, -z 32
 //   bits 6-7  0 update (flag 0), 3 update + boundary filter (interior flag 2),
 //             1 write zero (flag 1 and the domain border), 2 hard source (flag 3)
