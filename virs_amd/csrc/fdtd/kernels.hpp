@@ -968,6 +968,7 @@ struct TailParams {
     long long soft_len;
     int nrcv;
     const long long *rcv_idx, *rcv_bpos;
+    int list_points;         // some soft source or receiver sits on a list cell
     double *ir;              // forward: [nrcv][ir_cap] output
     const double *gsig;      // reverse: dL/dIR [nrcv][ir_cap]
     long long ir_cap;
@@ -1071,7 +1072,8 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
         r = boundary_update_pre<C, typename YsOf<T>::type>(p.bs, e, n, pp - qq, r, x1[u], x2[u], y1[u], y2[u]);
         T t = store_round<T>(r * sR, p.sr, (unsigned long long)b, n, kSaltBoundary);
         // a soft source on this cell is added to the stored corrected value
-        for (int k = 0; k < p.nsoft; ++k)
+        // (the point tables are scanned only when some point is on the list)
+        for (int k = 0; k < (p.list_points ? p.nsoft : 0); ++k)
             if (p.soft_bpos[k] == e && in_sig) {
                 C r2 = S::ld(t);
                 if constexpr (S::kScaled) r2 = r2 * sRi;
@@ -1080,7 +1082,7 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
             }
         R[b] = t;
         if (hvn) ((T *)p.hv[(n + 1) & 1])[e] = t;
-        if (in_ir)
+        if (in_ir && p.list_points)
             for (int k = 0; k < p.nrcv; ++k)
                 if (p.rcv_bpos[k] == e) {
                     C stored = S::ld(t);

@@ -71,6 +71,7 @@ struct virs_ctx {
 
     int nrcv = 0;
     long long *rcv_idx = nullptr, *rcv_bpos = nullptr;
+    bool points_on_list = false; // some soft source or receiver is a list cell
     std::vector<long long> h_rcv_idx;
     double *ir_buf = nullptr;
     long long ir_cap = 0;
@@ -181,8 +182,15 @@ long long bpos_of(const virs_ctx *c, long long idx)
 int upload_points(virs_ctx *c)
 {
     std::vector<long long> sb(c->nsoft), rb(c->nrcv);
-    for (int k = 0; k < c->nsoft; ++k) sb[k] = bpos_of(c, c->h_soft_idx[k]);
-    for (int k = 0; k < c->nrcv; ++k) rb[k] = bpos_of(c, c->h_rcv_idx[k]);
+    c->points_on_list = false;
+    for (int k = 0; k < c->nsoft; ++k) {
+        sb[k] = bpos_of(c, c->h_soft_idx[k]);
+        c->points_on_list |= sb[k] >= 0;
+    }
+    for (int k = 0; k < c->nrcv; ++k) {
+        rb[k] = bpos_of(c, c->h_rcv_idx[k]);
+        c->points_on_list |= rb[k] >= 0;
+    }
     if (c->nsoft) {
         HIPC(c, hipMemcpy(c->soft_idx, c->h_soft_idx.data(), sizeof(long long) * c->nsoft, hipMemcpyHostToDevice));
         HIPC(c, hipMemcpy(c->soft_bpos, sb.data(), sizeof(long long) * c->nsoft, hipMemcpyHostToDevice));
@@ -260,6 +268,7 @@ TailParams tail_params(const virs_ctx *c, const void *P, const void *Q, void *R,
     t.nrcv = c->nrcv;
     t.rcv_idx = c->rcv_idx;
     t.rcv_bpos = c->rcv_bpos;
+    t.list_points = c->points_on_list ? 1 : 0;
     t.ir = c->ir_buf;
     t.ir_cap = c->ir_cap;
     t.nx = (int)c->nx;
