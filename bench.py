@@ -90,11 +90,12 @@ def step_bytes(n, nb, nrcv, s):
     return dense_bytes_of(n, s) + nb * list_entry_bytes(s) + nrcv * s
 
 
-def run_gpu(prep, precision, steps, warmup, dense_blocks=0, fuse=False, vw=0, graph=True):
+def run_gpu(prep, precision, steps, warmup, dense_blocks=0, fuse=False, vw=0, graph=True,
+            rounding="stochastic"):
     from virs_amd.fdtd import BYTES, setup
     ts = time.perf_counter()
     f = setup(prep, precision, use_graph=graph, dense_blocks=dense_blocks, fuse_boundary=fuse,
-              vector_width=vw)
+              vector_width=vw, rounding=rounding)
     f.synchronize()
     setup_s = time.perf_counter() - ts
     f.start(warmup + steps)
@@ -344,7 +345,9 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "fp64"])
-    ap.add_argument("--variants", default="fp16", help="extra storage precisions to time, comma separated")
+    ap.add_argument("--variants", default="fp16",
+                    help="extra storage precisions to time, comma separated; fp16:nearest / bf16:nearest "
+                         "store with round-to-nearest instead of stochastic rounding")
     ap.add_argument("--dense-blocks", type=int, default=0, help="stencil grid size (0 = fill the GPU once)")
     ap.add_argument("--fuse", action="store_true", help="run the boundary filter inside the stencil pass")
     ap.add_argument("--vw16", type=int, default=0, help="vector width of the 16-bit variants (0 = auto)")
@@ -435,12 +438,16 @@ def main():
         "ir_finite": bool(np.isfinite(r["ir"]).all()),
     }
     variants = {}
-    for prec in [v for v in args.variants.split(",") if v and v != args.precision]:
+    for name in [v for v in args.variants.split(",") if v and v != args.precision]:
+        # a variant is a storage precision, optionally with ":nearest" for
+        # round-to-nearest storage instead of stochastic rounding
+        prec, _, rnd = name.partition(":")
         rv = run_gpu(prep, prec, args.steps, args.warmup, args.dense_blocks, args.fuse,
-                     args.vw16 if prec in ("fp16", "bf16") else 0, graph=not args.no_graph)
+                     args.vw16 if prec in ("fp16", "bf16") else 0, graph=not args.no_graph,
+                     rounding=rnd or "stochastic")
         vv = n * args.steps / (rv["total_ms"] / 1e3) / 1e9
         db = dense_bytes_of(n, rv["s"])
-        variants[prec] = {"value": round(vv, 3), "speedup_vs_main": round(vv / value, 3),
+        variants[name] = {"value": round(vv, 3), "speedup_vs_main": round(vv / value, 3),
                           "dense_gbs": round(db / (rv["dense_ms"] / 1e3) / 1e9, 1),
                           "dense_ms": round(rv["dense_ms"], 5), "boundary_kernel_ms": round(rv["bnd_ms"], 5)}
     if variants:
