@@ -141,10 +141,14 @@ constexpr uint32_t kSaltDense = 0u, kSaltBoundary = 0x68E31DA4u, kSaltSource = 0
 // x with the random bits r16 added below the 16-bit format's last place;
 // converting the result toward zero (cvt2_truncated) rounds x stochastically.
 template <typename T>
+constexpr uint32_t sr_mask()
+{
+    return __is_same(T, bf16) ? 0xffffu : 0x1fffu;
+}
+template <typename T>
 __device__ inline float sr_bits(float x, uint32_t r16)
 {
-    constexpr uint32_t mask = __is_same(T, bf16) ? 0xffffu : 0x1fffu;
-    return __uint_as_float(__float_as_uint(x) + (r16 & mask));
+    return __uint_as_float(__float_as_uint(x) + (r16 & sr_mask<T>()));
 }
 
 // Converts two floats to T rounding toward zero.
@@ -707,6 +711,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
         auto body = [&](const int z, C(&PM)[V], C(&PC)[V], C(&PP)[V], Stage &cur, Stage &nxt) {
             C qv[V], yn[V], ys[V];
             uint32_t cd[V];
+            // 16-bit: the kw fields of the even cells in the bytes of kwe,
+            // of the odd cells in the bytes of kwo, so each converts to float
+            // with one byte conversion (the empty asm keeps the compiler
+            // from folding the masks back into one extract per cell)
+            uint32_t kwe = 0u, kwo = 0u;
+            if constexpr (S::kScaled) {
+                kwe = cur.cd & 0x07070707u;
+                kwo = (cur.cd >> 4) & 0x07070707u;
+                asm volatile("" : "+v"(kwe), "+v"(kwo));
+            }
 #pragma unroll
             for (int j = 0; j < V; ++j) {
                 // at z = nz-1 this is plane z itself; that plane is border,
@@ -744,7 +758,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                     sum = sum + ys[j];
                     sum = sum + PP[j];
                     sum = sum + PM[j];
-                    const C lapc = (sum - (C)(c & 7u) * pcv) * c2ih2; // c & 7 = kw for the cells that update
+                    const C kw = (C)(((j & 1) ? kwo : kwe) >> (8 * (j >> 1)) & 0xffu); // cells that update
+                    const C lapc = (sum - kw * pcv) * c2ih2;
                     C a = (C)2, b = (C)1, g = (C)1;
                     if constexpr (SPONGE) {
                         const int d = edge_dist(x0 + j, y, z, nx, ny, nz);
@@ -811,9 +826,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                         const uint32_t lo = zlo * kHashK1 + c0k1;
 #pragma unroll
                         for (int j = 0; j < V; j += 2) {
-                            const uint32_t h = hash_pair(lo + (uint32_t)j * kHashK1, hk);
-                            const float qa = sr_bits<T>(o[j], h & 0xffffu);
-                            const float qb = sr_bits<T>(o[j + 1], h >> 16);
+                            // both halves masked at once (the same bits as
+                            // masking each 16-bit half); the empty asm keeps
+                            // the compiler from splitting the mask per half
+                            uint32_t h = hash_pair(lo + (uint32_t)j * kHashK1, hk) & (sr_mask<T>() * 0x10001u);
+                            asm volatile("" : "+v"(h));
+                            const float qa = __uint_as_float(__float_as_uint(o[j]) + (h & 0xffffu));
+                            const float qb = __uint_as_float(__float_as_uint(o[j + 1]) + (h >> 16));
                             cvt2_truncated<T>(qa, qb, out.v[j], out.v[j + 1]);
                         }
                     }
