@@ -661,14 +661,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
         const uint32_t on = oc + (uint32_t)dyn * (uint32_t)sizeof(T);
         const uint32_t os = oc - (uint32_t)dys * (uint32_t)sizeof(T);
         const uint32_t ob = (uint32_t)cell0;
-        // The empty asm makes an offset opaque where it is used, so its
-        // zero-extension stays next to the access and the access takes the
-        // scalar-base + 32-bit vector-offset form; otherwise the extension is
-        // hoisted out of the march and every access builds a 64-bit address.
-        auto lane_off = [](uint32_t o) {
-            asm volatile("" : "+v"(o));
-            return o;
-        };
+        // The offsets as used by the accesses.  An empty asm rewrites them
+        // in place at every plane (same values), so their zero-extension
+        // stays next to the accesses and those take the scalar-base + 32-bit
+        // vector-offset form; otherwise the extension is hoisted out of the
+        // march and every access builds a 64-bit address.  Rewriting the
+        // loop-carried copies themselves needs no register copies.
+        uint32_t xoc = oc, xon = on, xos = os, xob = S::kScaled ? ob >> 1 : ob;
         // stochastic-rounding hash: cell0's part of the pair index times K1
         const uint32_t c0k1 = ((uint32_t)cell0 & ~1u) * kHashK1;
 
@@ -676,19 +675,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
             const char *Pz = reinterpret_cast<const char *>(P + (long long)z * sxy);
             const char *Pzp = reinterpret_cast<const char *>(P + (long long)z * sxy + (z + 1 < nz ? sxy : 0));
             const char *Qz = reinterpret_cast<const char *>(Q + (long long)z * sxy);
-            const uint32_t xc = lane_off(oc);
+            asm volatile("" : "+v"(xoc), "+v"(xon), "+v"(xos), "+v"(xob));
+            const uint32_t xc = xoc;
             st.pp = vload<VT, false>(reinterpret_cast<const VT *>(Pzp + xc));
             st.q = vload<VT, true>(reinterpret_cast<const VT *>(Qz + xc));
             if constexpr (S::kScaled) {
                 if constexpr (V == 1)
                     st.cd = load_nibble1(p.code, (long long)z * sxy + cell0);
                 else // V divides Nx, so planes and vectors start on even cells
-                    st.cd = load_nibbles<V>(p.code + (((long long)z * sxy) >> 1), lane_off(ob >> 1));
+                    st.cd = load_nibbles<V>(p.code + (((long long)z * sxy) >> 1), xob);
             } else {
-                st.cd = vload<VB, true>(reinterpret_cast<const VB *>(p.code + (long long)z * sxy + lane_off(ob)));
+                st.cd = vload<VB, true>(reinterpret_cast<const VB *>(p.code + (long long)z * sxy + xob));
             }
-            st.yn = vload<VT, false>(reinterpret_cast<const VT *>(Pz + lane_off(on)));
-            st.ys = vload<VT, false>(reinterpret_cast<const VT *>(Pz + lane_off(os)));
+            st.yn = vload<VT, false>(reinterpret_cast<const VT *>(Pz + xon));
+            st.ys = vload<VT, false>(reinterpret_cast<const VT *>(Pz + xos));
             if (need_l) st.tl = reinterpret_cast<const T *>(Pz + xc)[-1];
             if (need_r) st.tr = reinterpret_cast<const T *>(Pz + xc)[V];
         };
@@ -844,7 +844,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
 #pragma unroll
                 for (int j = 0; j < V; ++j) out.v[j] = S::st(o[j]);
             }
-            if (active) vstore<VT, true>(reinterpret_cast<VT *>(reinterpret_cast<char *>(R + (long long)z * sxy) + lane_off(oc)), out);
+            if (active) vstore<VT, true>(reinterpret_cast<VT *>(reinterpret_cast<char *>(R + (long long)z * sxy) + xoc), out);
         };
 
         // planes rotate with period 3, load stages with period 2
