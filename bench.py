@@ -9,9 +9,10 @@ and 8 receivers, with fields already resident in HBM.  Prints one JSON line.
 roofline: the stencil launch's algorithmic bytes N*(3s+c) (read p^n and
 p^{n-1}, write p^{n+1}, read the cell code: c = 1 byte per cell, half a
 byte for 16-bit storage) divided by its mean launch time, taken with HIP
-events around the two launches of every 31st step of the timed region (the
-steps between are replayed as 30-step HIP graphs; --no-graph launches every
-step and samples about 64), on the library's own stream.
+events that the two launches of every 31st step of the timed region record
+at their own start and stop (hipExtLaunchKernel; the steps between are
+replayed as 30-step HIP graphs; --no-graph launches every step and samples
+about 64), on the library's own stream.
 step_hbm_frac: the whole step's algorithmic bytes (stencil, list launch,
 receivers) per step time; step_hbm_frac_fp64_state_formula prices the same
 step with N(3s+1) + Nb(69+4s) + R s instead.

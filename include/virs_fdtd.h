@@ -161,9 +161,11 @@ int virs_memory_bytes(const virs_ctx *ctx, int64_t *bytes);
 
 /* Runs nsteps and times them on the context's stream with HIP events:
  * total_ms over the whole run, and the mean duration of the stencil launch
- * and of the boundary/point launch, taken with events around the launches of
- * a sample of the steps (with use_graph: every 31st step, the others replayed
- * as 30-step graphs; otherwise about 64 steps spread over the run). */
+ * and of the boundary/point launch over a sample of the steps (with
+ * use_graph: every 31st step, the others replayed as 30-step graphs;
+ * otherwise about 64 steps spread over the run).  A sampled launch records
+ * its own start and stop (hipExtLaunchKernel), so the durations exclude the
+ * dispatch around the kernels. */
 int virs_time_steps(virs_ctx *ctx, int64_t nsteps, double *total_ms, double *dense_ms,
                     double *boundary_ms);
 

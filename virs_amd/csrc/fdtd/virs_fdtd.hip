@@ -9,6 +9,8 @@
 #include "kernels.hpp"
 #include "virs_fdtd.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
@@ -84,6 +86,10 @@ struct virs_ctx {
 
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
     int graph_steps = 0;
+    // when set, the next stencil or list launch records its own start and
+    // stop in these events (hipExtLaunchKernel: the kernel's execution, not
+    // the dispatch around it); launch_step sets them for sampled steps
+    hipEvent_t kev[2] = {nullptr, nullptr};
 
     // adjoint
     int ncp = 0;
@@ -323,7 +329,10 @@ hipError_t launch_dense_k(const virs_ctx *c, const DenseParams &p)
     auto k = k_dense<T, V, ADJ, FUSED, SPONGE, SR>;
     DenseParams q = p;
     const int grid = dense_grid(c, k, q.zranges);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, c->stream, q);
+    if (c->kev[0])
+        hipExtLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, c->stream, c->kev[0], c->kev[1], 0, q);
+    else
+        hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, c->stream, q);
     return hipGetLastError();
 }
 
@@ -387,7 +396,10 @@ void launch_tail_t(const virs_ctx *c, const TailParams &t)
 {
     const long long per = (long long)kBlock * tail_batch<T>();
     const int g = (int)std::max<long long>(1, (t.bs.nb + per - 1) / per);
-    hipLaunchKernelGGL(k_tail<T>, dim3(g), dim3(kBlock), 0, c->stream, t);
+    if (c->kev[0])
+        hipExtLaunchKernelGGL(k_tail<T>, dim3(g), dim3(kBlock), 0, c->stream, c->kev[0], c->kev[1], 0, t);
+    else
+        hipLaunchKernelGGL(k_tail<T>, dim3(g), dim3(kBlock), 0, c->stream, t);
 }
 
 hipError_t launch_tail(const virs_ctx *c, const TailParams &t)
@@ -460,11 +472,14 @@ int launch_step(virs_ctx *c, void *trace_d = nullptr, void *trace_y = nullptr,
         t.rcv_bpos = c->neg;
     }
     if (!receivers) t.nrcv = 0;
-    if (ev) HIPC(c, hipEventRecord(ev[0], c->stream));
-    HIPC(c, launch_dense(c, p, false));
-    if (ev) HIPC(c, hipEventRecord(ev[1], c->stream));
-    HIPC(c, launch_tail(c, t));
-    if (ev) HIPC(c, hipEventRecord(ev[2], c->stream));
+    // ev (4 events, may be null): start and stop of the stencil launch, then
+    // of the list launch
+    if (ev) c->kev[0] = ev[0], c->kev[1] = ev[1];
+    hipError_t e = launch_dense(c, p, false);
+    if (ev) c->kev[0] = ev[2], c->kev[1] = ev[3];
+    if (e == hipSuccess) e = launch_tail(c, t);
+    c->kev[0] = c->kev[1] = nullptr;
+    HIPC(c, e);
     const int ip = c->ip;
     c->ip = c->ir;
     c->ir = c->iq;
@@ -1303,11 +1318,12 @@ int virs_time_steps(virs_ctx *c, int64_t nsteps, double *total_ms, double *dense
     int rc = check_ready(c);
     if (rc) return rc;
     if (nsteps <= 0) return VIRS_ERR_ARG;
-    // Launch durations are taken on a sample of the steps (events around its
-    // two launches).  With graphs, every sampled step is followed by one
-    // replay of a 30-step graph; otherwise about 64 steps spread over the run
-    // are sampled and the others launched plainly.  Either way the event
-    // records barely touch the loop time.
+    // Launch durations are taken on a sample of the steps: each of its two
+    // launches records its own start and stop (hipExtLaunchKernel), which
+    // excludes the dispatch around it.  With graphs, every sampled step is
+    // followed by one replay of a 30-step graph; otherwise about 64 steps
+    // spread over the run are sampled and the others launched plainly.
+    // Either way the sampling barely touches the loop time.
     const int gsteps = 30;
     const bool graphs = c->d.use_graph && nsteps > gsteps;
     if (graphs) {
@@ -1318,7 +1334,7 @@ int virs_time_steps(virs_ctx *c, int64_t nsteps, double *total_ms, double *dense
     if (rc) return rc;
     const long long stride = graphs ? gsteps + 1 : std::max<long long>(1, nsteps / 64);
     const long long nsamp = (nsteps + stride - 1) / stride;
-    std::vector<hipEvent_t> ev((size_t)nsamp * 3);
+    std::vector<hipEvent_t> ev((size_t)nsamp * 4);
     for (auto &e : ev) HIPC(c, hipEventCreate(&e));
     hipEvent_t t0, t1;
     HIPC(c, hipEventCreate(&t0));
@@ -1327,7 +1343,7 @@ int virs_time_steps(virs_ctx *c, int64_t nsteps, double *total_ms, double *dense
     HIPC(c, hipEventRecord(t0, c->stream));
     long long s = 0, k = 0;
     while (s < nsteps && !rc) {
-        rc = launch_step(c, nullptr, nullptr, 0, true, &ev[(size_t)k * 3]);
+        rc = launch_step(c, nullptr, nullptr, 0, true, &ev[(size_t)k * 4]);
         ++k;
         ++s;
         ++c->steps;
@@ -1353,8 +1369,8 @@ int virs_time_steps(virs_ctx *c, int64_t nsteps, double *total_ms, double *dense
     double dsum = 0, bsum = 0;
     for (long long j = 0; j < k && !rc; ++j) {
         float a = 0, b = 0;
-        HIPC(c, hipEventElapsedTime(&a, ev[j * 3], ev[j * 3 + 1]));
-        HIPC(c, hipEventElapsedTime(&b, ev[j * 3 + 1], ev[j * 3 + 2]));
+        HIPC(c, hipEventElapsedTime(&a, ev[j * 4], ev[j * 4 + 1]));
+        HIPC(c, hipEventElapsedTime(&b, ev[j * 4 + 2], ev[j * 4 + 3]));
         dsum += a;
         bsum += b;
     }
