@@ -346,7 +346,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "fp64"])
-    ap.add_argument("--variants", default="fp16",
+    ap.add_argument("--variants", default="fp16,bf16",
                     help="extra storage precisions to time, comma separated; fp16:nearest / bf16:nearest "
                          "store with round-to-nearest instead of stochastic rounding")
     ap.add_argument("--dense-blocks", type=int, default=0, help="stencil grid size (0 = fill the GPU once)")
