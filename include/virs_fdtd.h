@@ -173,9 +173,14 @@ int virs_time_steps(virs_ctx *ctx, int64_t nsteps, double *total_ms, double *den
  * (b0,b1,b2,a1,a2); no reference counterpart.
  * virs_forward_record runs nsteps from zero state keeping ncheckpoints
  * snapshots (p^n, p^{n-1}, filter state); the IR is then readable as usual.
+ * With ncheckpoints = 1 the whole boundary trace (2 values per list entry
+ * and step) is recorded and nothing is recomputed; ncheckpoints = 0 picks
+ * that when the trace takes at most a quarter of the free device memory,
+ * else ceil(sqrt(nsteps)) checkpoints.
  * virs_adjoint takes dL/dIR [nrcv][nsteps] and writes grad[nmat][5]; it
  * recomputes each segment forward from its snapshot and runs the
- * time-reversed stencil with the transposed boundary filters.  FP32/FP64. */
+ * time-reversed stencil with the transposed boundary filters.  The gradient
+ * does not depend on the number of checkpoints (bit for bit).  FP32/FP64. */
 int virs_forward_record(virs_ctx *ctx, int64_t nsteps, int32_t ncheckpoints);
 int virs_adjoint(virs_ctx *ctx, const float *dl_dir, double *grad);
 

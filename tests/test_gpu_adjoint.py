@@ -51,8 +51,11 @@ def test_fp32_adjoint_within_1e4_of_fp64(gpu):
     f32.close()
 
 
-@pytest.mark.parametrize("ncp", [1, 2, 7, 100])
+@pytest.mark.parametrize("ncp", [0, 1, 2, 7, 100])
 def test_checkpoint_count_does_not_change_the_gradient(gpu, ncp):
+    """The whole trace (one segment), any number of checkpoints, and the
+    automatic choice (0) give the same gradient bit for bit, also when the
+    trace buffers are reused from a run with another segment length."""
     p = tiny_problem(23, dims=(18, 14, 12), steps=100)
     n = p["steps"]
     dl = np.random.default_rng(3).standard_normal((len(p["rcv_idx"]), n)).astype(np.float32)
@@ -62,6 +65,8 @@ def test_checkpoint_count_does_not_change_the_gradient(gpu, ncp):
     f.forward_record(n, ncp)
     g = f.adjoint(dl)
     np.testing.assert_array_equal(g, ref)
+    f.forward_record(n, 1)
+    np.testing.assert_array_equal(f.adjoint(dl), ref)
     f.close()
 
 

@@ -1287,7 +1287,7 @@ int virs_memory_bytes(const virs_ctx *c, int64_t *bytes)
     b += (long long)sizeof(double) * std::max(1, c->nrcv) * std::max<long long>(1, c->ir_cap);
     b += (long long)c->cp_field.size() * (long long)c->esz * c->N;
     b += (long long)c->cp_state.size() * (2 * (long long)compute_bytes(c) + 16) * c->nb;
-    b += 4 * (long long)compute_bytes(c) * c->trace_slots * c->nb;
+    b += (c->trace_d[1] ? 4 : 2) * (long long)compute_bytes(c) * c->trace_slots * c->nb; // d and y per buffer
     for (auto p : c->af)
         if (p) b += (long long)c->esz * c->N;
     if (c->snap_host) b += (long long)kSnapSlots * c->nx * c->nz * (long long)sizeof(float);
@@ -1409,9 +1409,11 @@ struct OnStream {
     ~OnStream() { c->stream = saved; }
 };
 
-static int ensure_trace(virs_ctx *c, long long slots)
+// Trace buffers of `slots` steps: one when a single segment is recorded
+// whole (nothing is recomputed), two when recomputation overlaps the sweep.
+static int ensure_trace(virs_ctx *c, long long slots, int nbuf)
 {
-    if (c->trace_slots >= slots && c->trace_d[0] && c->trace_d[1]) return VIRS_OK;
+    if (c->trace_slots >= slots && c->trace_d[0] && (nbuf == 1 || c->trace_d[1])) return VIRS_OK;
     for (int k = 0; k < 2; ++k) {
         dfree(c->trace_d[k]);
         dfree(c->trace_y[k]);
@@ -1419,7 +1421,7 @@ static int ensure_trace(virs_ctx *c, long long slots)
     c->trace_slots = 0;
     const size_t bytes = compute_bytes(c) * slots * std::max<long long>(1, c->nb);
     int rc = VIRS_OK;
-    for (int k = 0; k < 2 && !rc; ++k) {
+    for (int k = 0; k < nbuf && !rc; ++k) {
         rc = dalloc(c, &c->trace_d[k], bytes);
         if (!rc) rc = dalloc(c, &c->trace_y[k], bytes);
     }
@@ -1494,8 +1496,18 @@ int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
     if (c->d.precision != VIRS_FP32 && c->d.precision != VIRS_FP64)
         return fail(c, VIRS_ERR_UNSUPPORTED, "adjoint needs FP32 or FP64 fields");
     if (c->nmat > 8) return fail(c, VIRS_ERR_UNSUPPORTED, "adjoint supports up to 8 materials");
-    if (nsteps <= 0 || ncp <= 0) return VIRS_ERR_ARG;
+    if (nsteps <= 0 || ncp < 0) return VIRS_ERR_ARG;
     if (c->ir_cap < nsteps) return fail(c, VIRS_ERR_STATE, "receiver buffer shorter than nsteps");
+    if (ncp == 0) {
+        // automatic: record the whole boundary trace (no recomputation) when
+        // it takes at most a quarter of the free device memory, else sqrt(N)
+        // checkpoints
+        size_t free_b = 0, total_b = 0;
+        HIPC(c, hipMemGetInfo(&free_b, &total_b));
+        const double whole = 2.0 * (double)compute_bytes(c) * (double)(nsteps + 2) *
+                             (double)std::max<long long>(1, c->nb);
+        ncp = whole <= 0.25 * (double)free_b ? 1 : (int)std::ceil(std::sqrt((double)nsteps));
+    }
     ncp = (int)std::min<long long>(ncp, nsteps);
     c->seg = (nsteps + ncp - 1) / ncp;
     ncp = (int)((nsteps + c->seg - 1) / c->seg);
@@ -1513,7 +1525,7 @@ int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
         if (rc) return rc;
     }
     c->ncp = ncp;
-    rc = ensure_trace(c, c->seg + 2);
+    rc = ensure_trace(c, c->seg + 2, ncp == 1 ? 1 : 2);
     if (rc) return rc;
     rc = device_early_stop(c, false);
     if (!rc) rc = reset_run(c);
