@@ -1175,6 +1175,10 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
     const BoundaryState &bs = p.bs;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const long long e = (long long)blockIdx.x * kBlock + threadIdx.x;
+    // this block's running sums, loaded now so the load is not the last link
+    // of the block's chain (the sum below adds in the same order)
+    double *dst = p.grad_part + (long long)blockIdx.x * 5 * bs.nmat + threadIdx.x;
+    const double prev = threadIdx.x < 5 * bs.nmat ? *dst : 0.0;
     double gv[5] = {0, 0, 0, 0, 0};
     int m = -1;
     if (e < bs.nb) {
@@ -1192,7 +1196,7 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
         const double *dbr0 = (const double *)bs.xs[sp ^ 1], *dbr1 = (const double *)bs.xs[sp];
         const double yb1 = ((const double *)bs.ys[sp ^ 1])[e], yb2 = ((const double *)bs.ys[sp])[e], db1 = dbr0[e], db2 = dbr1[e];
         rb = rb + db1 - db2;
-        if (in_g)
+        if (in_g && p.list_points)
             for (int k = 0; k < p.nrcv; ++k)
                 if (p.rcv_bpos[k] == e) rb += p.gsig[(long long)k * p.ir_cap + n];
         const double kap = (double)bs.kconst * (double)0.4f;
@@ -1252,8 +1256,7 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
     if (threadIdx.x < 5 * bs.nmat) {
         double s = part[0][threadIdx.x];
         for (int w = 1; w < kWaves; ++w) s = s + part[w][threadIdx.x];
-        double *dst = p.grad_part + (long long)blockIdx.x * 5 * bs.nmat + threadIdx.x;
-        *dst = *dst + s;
+        *dst = prev + s;
     }
 }
 
