@@ -364,7 +364,7 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "fp64"])
-    ap.add_argument("--variants", default="fp16,bf16",
+    ap.add_argument("--variants", default="fp16,bf16,fp64",
                     help="extra storage precisions to time, comma separated; fp16:nearest / bf16:nearest "
                          "store with round-to-nearest instead of stochastic rounding")
     ap.add_argument("--dense-blocks", type=int, default=0, help="stencil grid size (0 = fill the GPU once)")
@@ -468,6 +468,7 @@ def main():
         db = dense_bytes_of(n, rv["s"])
         variants[name] = {"value": round(vv, 3), "speedup_vs_main": round(vv / value, 3),
                           "dense_gbs": round(db / (rv["dense_ms"] / 1e3) / 1e9, 1),
+                          "dense_frac": round(db / (rv["dense_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
                           "dense_ms": round(rv["dense_ms"], 5), "boundary_kernel_ms": round(rv["bnd_ms"], 5)}
     if variants:
         out["variants"] = variants
