@@ -407,7 +407,7 @@ __device__ inline int edge_dist(int x, int y, int z, int nx, int ny, int nz)
 
 struct BoundaryState {
     long long nb;
-    const uint8_t *bmat;       // may be null: row 0
+    const uint8_t *bmat;       // material row per entry (zeros for one material)
     // Filter state as two-slot rings indexed by step parity: on entry to
     // step n, slot n&1 holds the value of step n-2 and slot (n+1)&1 that of
     // step n-1; step n overwrites slot n&1, so a step writes one value per
@@ -489,24 +489,43 @@ __device__ inline void load_taps(const BoundaryState &bs, long long e, long long
 //   y = b0 d + b1 x1 + b2 x2 - a1 y1 - a2 y2   (in double; stored as Y)
 //   R = R - K (0.1 y + 0.9 d) 0.4f             (evaluated in double)
 // x1..y2 is the entry's state as loaded before the call.
-template <typename C, typename Y>
-__device__ inline C boundary_update_pre(const BoundaryState &bs, long long e, long long n, C dif, C r,
-                                        double x1, double x2, double y1, double y2)
+// The filter output and the corrected pressure, the arithmetic shared by the
+// list launch and the fused stencil phase (one expression order for both).
+__device__ inline double biquad_out(const double *cf, double in, double x1, double x2, double y1, double y2)
 {
-    const int m = bs.bmat ? bs.bmat[e] : 0;
-    const double *cf = bs.coeffs + 5 * m;
-    const double in = (double)dif;
-    const double yv = cf[0] * in + cf[1] * x1 + cf[2] * x2 - cf[3] * y1 - cf[4] * y2;
+    return cf[0] * in + cf[1] * x1 + cf[2] * x2 - cf[3] * y1 - cf[4] * y2;
+}
+template <typename C>
+__device__ inline C corrected(C r, float kconst, double yv, double in)
+{
+    return (C)((double)r - (double)kconst * (yv * 0.1 + in * 0.9) * (double)0.4f);
+}
+
+// Stores of entry e at step n: the new filter taps and, when recording,
+// the trace of the adjoint's segment.
+template <typename C, typename Y>
+__device__ inline void store_taps(const BoundaryState &bs, long long e, long long n, C dif, double yv)
+{
     ((C *)bs.xs[n & 1])[e] = dif;
     ((Y *)bs.ys[n & 1])[e] = (Y)yv;
     if (bs.trace_d) {
         const long long slot = n - bs.trace_base + 2;
         if (slot >= 0 && slot < bs.trace_slots) {
-            ((C *)bs.trace_d)[slot * bs.nb + e] = (C)in;
+            ((C *)bs.trace_d)[slot * bs.nb + e] = (C)(double)dif;
             ((C *)bs.trace_y)[slot * bs.nb + e] = (C)yv;
         }
     }
-    return (C)((double)r - (double)bs.kconst * (yv * 0.1 + in * 0.9) * (double)0.4f);
+}
+
+template <typename C, typename Y>
+__device__ inline C boundary_update_pre(const BoundaryState &bs, long long e, long long n, C dif, C r,
+                                        double x1, double x2, double y1, double y2)
+{
+    const int m = bs.bmat ? bs.bmat[e] : 0;
+    const double in = (double)dif;
+    const double yv = biquad_out(bs.coeffs + 5 * m, in, x1, x2, y1, y2);
+    store_taps<C, Y>(bs, e, n, dif, yv);
+    return corrected<C>(r, bs.kconst, yv, in);
 }
 
 template <typename C, typename Y>
@@ -1016,98 +1035,85 @@ __device__ inline void sponge_g(const TailParams &p, long long idx, C &g)
 // list when it was not fused into the stencil, then soft sources (added to
 // the corrected field), receiver gather, and the step counter.  Point cells
 // on the list are handled by their entry's thread, the others by block 0.
-// A thread handles tail_batch<T>() entries (block b covers entries
-// [b*U*kBlock, (b+1)*U*kBlock), lane-contiguous per batch slot) and issues
-// all their loads before any store: the entries are distinct cells, so no
-// store of the batch feeds a load of it.  p^n and p^{n-1} come from the
-// coalesced ring hv instead of two scattered gathers.
-#ifndef VIRS_TAIL_BATCH16
-#define VIRS_TAIL_BATCH16 1 // entries per thread with 16-bit fields (measured)
-#endif
-#ifndef VIRS_TAIL_BATCH32
-#define VIRS_TAIL_BATCH32 1 // entries per thread with FP32/FP64 fields (measured)
-#endif
-template <typename T>
-constexpr int tail_batch()
-{
-    return Store<T>::kScaled ? VIRS_TAIL_BATCH16 : VIRS_TAIL_BATCH32;
-}
+//
+// One entry per thread (two or four per thread were slower).  The launch is
+// bound by latency, not bytes, so every load that the entry index e names
+// (the list index, the material byte, both slots of the pressure ring hv and
+// of the filter taps) is issued at once, from e clamped into the list, and
+// none depends on the step counter: both ring slots are read every step, and
+// the counter only picks which slot is which.  What is left in series is
+// the list index, then the gather of p^{n+1} at the entry's cell; the
+// material's coefficients (cached) follow the material byte in parallel.
+// No load sits behind a branch, since a guarded load makes the compiler wait
+// for it where the branch rejoins.  The stores are predicated on the entry
+// being in the list and the step not skipped (stop bit).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
 {
     using S = Store<T>;
     using C = typename S::C;
-    constexpr int U = tail_batch<T>();
+    using Y = typename YsOf<T>::type;
     const long long n = p.ctl->nring[p.slot];
-    if (n & kStopBit) {
-        // a skipped step still passes the counter (and its stop bit) on
-        if (blockIdx.x == 0 && threadIdx.x == 0) p.ctl->nring[p.next_slot] = n + 1;
-        return;
-    }
+    const bool skip = (n & kStopBit) != 0;
     C sP = (C)1, sQ = (C)1, sR = (C)1, sRi = (C)1;
     if constexpr (S::kScaled) {
-        sP = exp2i(-p.ctl->exp[(n - 1) & 3]);
-        sQ = exp2i(-p.ctl->exp[(n - 2) & 3]);
-        const int eR = p.ctl->exp[n & 3];
+        // all four exponents in one load, selected by the step
+        const int4 ex = *(const int4 *)p.ctl->exp;
+        const int ring[4] = {ex.x, ex.y, ex.z, ex.w};
+        sP = exp2i(-ring[(n - 1) & 3]);
+        sQ = exp2i(-ring[(n - 2) & 3]);
+        const int eR = ring[n & 3];
         sR = exp2i(eR);
         sRi = exp2i(-eR);
     }
-    const T *__restrict__ P = (const T *)p.P;
-    const T *__restrict__ Q = (const T *)p.Q;
     T *__restrict__ R = (T *)p.R;
     const bool in_sig = n >= 0 && n < p.soft_len;
     const bool in_ir = n >= 0 && n < p.ir_cap;
-
-    long long ev[U], bv[U];
-    bool ok[U];
-    C pv[U], qv[U], rv[U];
-    double x1[U], x2[U], y1[U], y2[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        ev[u] = ((long long)blockIdx.x * U + u) * kBlock + threadIdx.x;
-        ok[u] = ev[u] < p.bs.nb;
-        bv[u] = ok[u] ? (long long)p.bidx[ev[u]] : 0;
-    }
-    const T *hvn = (const T *)p.hv[n & 1], *hvq = (const T *)p.hv[(n + 1) & 1];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        if (!ok[u]) continue;
-        if (hvn) {
-            pv[u] = S::ld(hvn[ev[u]]);
-            qv[u] = S::ld(hvq[ev[u]]);
-        } else {
-            pv[u] = S::ld(P[bv[u]]);
-            qv[u] = S::ld(Q[bv[u]]);
-        }
-        rv[u] = S::ld(R[bv[u]]);
-        load_taps<C, typename YsOf<T>::type>(p.bs, ev[u], n, x1[u], x2[u], y1[u], y2[u]);
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-        if (!ok[u]) continue;
-        const long long e = ev[u], b = bv[u];
-        C pp = pv[u], qq = qv[u], r = rv[u];
+    const long long nb = p.bs.nb;
+    if (nb > 0) {
+        const long long e = (long long)blockIdx.x * kBlock + threadIdx.x;
+        const long long ec = e < nb ? e : nb - 1;
+        const uint32_t b = p.bidx[ec];
+        const uint8_t m = p.bs.bmat[ec]; // always present for the list launch
+        const T h0 = ((const T *)p.hv[0])[ec], h1 = ((const T *)p.hv[1])[ec];
+        const C xa = ((const C *)p.bs.xs[0])[ec], xb = ((const C *)p.bs.xs[1])[ec];
+        const Y ya = ((const Y *)p.bs.ys[0])[ec], yb = ((const Y *)p.bs.ys[1])[ec];
+        const T rt = R[b];
+        const double *cf = p.bs.coeffs + 5 * m;
+        // before step n, slot n&1 holds step n-2's value and slot (n+1)&1
+        // step n-1's (the ring rule of BoundaryState and hv)
+        const bool odd = (n & 1) != 0;
+        C pp = S::ld(odd ? h1 : h0), qq = S::ld(odd ? h0 : h1), r = S::ld(rt);
+        const double x1 = (double)(odd ? xa : xb), x2 = (double)(odd ? xb : xa);
+        const double y1 = (double)(odd ? ya : yb), y2 = (double)(odd ? yb : ya);
         if constexpr (S::kScaled) pp = pp * sP, qq = qq * sQ, r = r * sRi;
-        r = boundary_update_pre<C, typename YsOf<T>::type>(p.bs, e, n, pp - qq, r, x1[u], x2[u], y1[u], y2[u]);
+        const C dif = pp - qq;
+        const double in = (double)dif;
+        const double yv = biquad_out(cf, in, x1, x2, y1, y2);
+        r = corrected<C>(r, p.bs.kconst, yv, in);
         T t = store_round<T>(r * sR, p.sr, (unsigned long long)b, n, kSaltBoundary);
-        // a soft source on this cell is added to the stored corrected value
-        // (the point tables are scanned only when some point is on the list)
-        for (int k = 0; k < (p.list_points ? p.nsoft : 0); ++k)
-            if (p.soft_bpos[k] == e && in_sig) {
-                C r2 = S::ld(t);
-                if constexpr (S::kScaled) r2 = r2 * sRi;
-                r2 = r2 + (C)p.soft_sig[(long long)k * p.soft_len + n];
-                t = store_round<T>(r2 * sR, p.sr, (unsigned long long)b, n, kSaltSource);
-            }
-        R[b] = t;
-        if (hvn) ((T *)p.hv[(n + 1) & 1])[e] = t;
-        if (in_ir && p.list_points)
-            for (int k = 0; k < p.nrcv; ++k)
-                if (p.rcv_bpos[k] == e) {
-                    C stored = S::ld(t);
-                    if constexpr (S::kScaled) stored = stored * sRi;
-                    p.ir[(long long)k * p.ir_cap + n] = (double)stored;
+        if (e < nb && !skip) {
+            store_taps<C, Y>(p.bs, e, n, dif, yv);
+            // a soft source on this cell is added to the stored corrected
+            // value (the point tables are scanned only when some point is on
+            // the list)
+            for (int k = 0; k < (p.list_points ? p.nsoft : 0); ++k)
+                if (p.soft_bpos[k] == e && in_sig) {
+                    C r2 = S::ld(t);
+                    if constexpr (S::kScaled) r2 = r2 * sRi;
+                    r2 = r2 + (C)p.soft_sig[(long long)k * p.soft_len + n];
+                    t = store_round<T>(r2 * sR, p.sr, (unsigned long long)b, n, kSaltSource);
                 }
+            R[b] = t;
+            ((T *)p.hv[(n + 1) & 1])[e] = t;
+            if (in_ir && p.list_points)
+                for (int k = 0; k < p.nrcv; ++k)
+                    if (p.rcv_bpos[k] == e) {
+                        C stored = S::ld(t);
+                        if constexpr (S::kScaled) stored = stored * sRi;
+                        p.ir[(long long)k * p.ir_cap + n] = (double)stored;
+                    }
+        }
     }
     if (blockIdx.x == 0) {
         const int k = threadIdx.x;

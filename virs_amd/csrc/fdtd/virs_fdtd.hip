@@ -394,8 +394,7 @@ hipError_t launch_dense(const virs_ctx *c, const DenseParams &p, bool adj)
 template <typename T>
 void launch_tail_t(const virs_ctx *c, const TailParams &t)
 {
-    const long long per = (long long)kBlock * tail_batch<T>();
-    const int g = (int)std::max<long long>(1, (t.bs.nb + per - 1) / per);
+    const int g = tail_grid(t.bs.nb);
     if (c->kev[0])
         hipExtLaunchKernelGGL(k_tail<T>, dim3(g), dim3(kBlock), 0, c->stream, c->kev[0], c->kev[1], 0, t);
     else
@@ -819,10 +818,13 @@ int virs_upload_boundary(virs_ctx *c, const uint32_t *idx, const uint8_t *mat, i
     int rc = dalloc(c, &c->bidx, sizeof(uint32_t) * nb);
     for (int k = 0; k < 4 && !rc; ++k) rc = dalloc(c, &c->st[k], state_bytes(c, k));
     for (int k = 0; k < 2 && !rc; ++k) rc = dalloc(c, &c->hv[k], c->esz * (size_t)std::max<long long>(1, nb));
-    if (!rc && mat) rc = dalloc(c, &c->bmat, (size_t)nb);
+    // the material array always exists on the device (zeros without one):
+    // the list launch reads it unconditionally
+    if (!rc) rc = dalloc(c, &c->bmat, (size_t)std::max<long long>(1, nb));
     if (rc) return rc;
     if (nb) HIPC(c, hipMemcpy(c->bidx, idx, sizeof(uint32_t) * nb, hipMemcpyHostToDevice));
     if (nb && mat) HIPC(c, hipMemcpy(c->bmat, mat, (size_t)nb, hipMemcpyHostToDevice));
+    if (nb && !mat) HIPC(c, hipMemset(c->bmat, 0, (size_t)nb));
     for (int k = 0; k < 4; ++k)
         if (nb) HIPC(c, hipMemset(c->st[k], 0, state_bytes(c, k)));
     rc = stash_boundary(c, c->steps);
@@ -1283,7 +1285,7 @@ int virs_memory_bytes(const virs_ctx *c, int64_t *bytes)
     if (!c || !bytes) return VIRS_ERR_ARG;
     long long b = 3 * (long long)c->esz * c->N + (long long)code_bytes(c); // fields + cell codes
     if (c->normals) b += c->N;
-    b += c->nb * (4 + 2 * (long long)compute_bytes(c) + 16 + 2 * (long long)c->esz + (c->bmat ? 1 : 0));
+    b += c->nb * (4 + 2 * (long long)compute_bytes(c) + 16 + 2 * (long long)c->esz + 1);
     b += (long long)sizeof(double) * std::max(1, c->nrcv) * std::max<long long>(1, c->ir_cap);
     b += (long long)c->cp_field.size() * (long long)c->esz * c->N;
     b += (long long)c->cp_state.size() * (2 * (long long)compute_bytes(c) + 16) * c->nb;
