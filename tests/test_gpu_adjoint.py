@@ -33,6 +33,32 @@ def test_fp64_adjoint_matches_oracle(gpu, sponge):
     f.close()
 
 
+def test_fp64_adjoint_receivers_sharing_cells(gpu):
+    """Receivers that share a cell, off the list (three on one cell) and on
+    it (two): each receiver's dL/dIR enters the adjoint field once, as the
+    oracle's in-order adds do."""
+    p = tiny_problem(22, dims=(20, 16, 12), steps=70, sponge=3)
+    r = p["rcv_idx"]
+    bidx = p["bidx"].astype(np.int64)
+    on_list = int(bidx[len(bidx) // 2])
+    p["rcv_idx"] = np.array([r[0], r[1], r[0], on_list, r[2], on_list, r[0]], np.int64)
+    n = p["steps"]
+    _say = lambda m: print(m, flush=True)
+    _say("fp64 adjoint, receivers sharing cells")
+    f = gpu_context(p, "fp64")
+    f.forward_record(n, 4)
+    ir = f.receivers(True)
+    np.testing.assert_array_equal(ir[2], ir[0])
+    np.testing.assert_array_equal(ir[5], ir[3])
+    target = np.random.default_rng(5).standard_normal(ir.shape) * 1e-3
+    dl = (2 * (ir - target)).astype(np.float32)
+    g = f.adjoint(dl)
+    g_or = oracle_problem(p).adjoint(n, dl.astype(np.float64))
+    print(f"fp64 adjoint (shared receiver cells) vs oracle rel {rel_l2(g, g_or):.2e}", flush=True)
+    assert rel_l2(g, g_or) <= 1e-10
+    f.close()
+
+
 def test_fp32_adjoint_within_1e4_of_fp64(gpu):
     p = tiny_problem(22, dims=(22, 16, 14), steps=100, sponge=2)
     n = p["steps"]

@@ -1006,6 +1006,10 @@ struct TailParams {
     long long soft_len;
     int nrcv;
     const long long *rcv_idx, *rcv_bpos;
+    // per receiver k: the soft source on its cell (-1: none), the next
+    // receiver on the same cell (-1: none), 1 if no earlier receiver shares
+    // it; per soft source: the first receiver on its cell (-1: none)
+    const int *rcv_src, *rcv_next, *rcv_first, *src_rcv;
     int list_points;         // some soft source or receiver sits on a list cell
     double *ir;              // forward: [nrcv][ir_cap] output
     const double *gsig;      // reverse: dL/dIR [nrcv][ir_cap]
@@ -1027,6 +1031,23 @@ __device__ inline void sponge_g(const TailParams &p, long long idx, C &g)
         const int y = (int)(r / p.nx);
         const int x = (int)(r - (long long)y * p.nx);
         const int d = edge_dist(x, y, z, p.nx, p.ny, p.nz);
+        if (d < p.sp.w) g = (C)p.sp.g[d];
+    }
+}
+
+// sponge_g for a list cell: list indices are 32-bit, so the coordinates
+// come from 32-bit divisions (exact, and far shorter than 64-bit ones).
+template <typename C>
+__device__ inline void sponge_g32(const TailParams &p, uint32_t idx, C &g)
+{
+    g = (C)1;
+    if (p.sp.w > 0) {
+        const uint32_t nx = (uint32_t)p.nx, sxy = nx * (uint32_t)p.ny;
+        const uint32_t z = idx / sxy;
+        const uint32_t r = idx - z * sxy;
+        const uint32_t y = r / nx;
+        const uint32_t x = r - y * nx;
+        const int d = edge_dist((int)x, (int)y, (int)z, p.nx, p.ny, p.nz);
         if (d < p.sp.w) g = (C)p.sp.g[d];
     }
 }
@@ -1070,8 +1091,8 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
     const bool in_sig = n >= 0 && n < p.soft_len;
     const bool in_ir = n >= 0 && n < p.ir_cap;
     const long long nb = p.bs.nb;
-    if (nb > 0) {
-        const long long e = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (blockIdx.x > 0) { // list blocks exist only when nb > 0
+        const long long e = (long long)(blockIdx.x - 1) * kBlock + threadIdx.x;
         const long long ec = e < nb ? e : nb - 1;
         const uint32_t b = p.bidx[ec];
         const uint8_t m = p.bs.bmat[ec]; // always present for the list launch
@@ -1114,51 +1135,67 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
                         p.ir[(long long)k * p.ir_cap + n] = (double)stored;
                     }
         }
+        return;
     }
-    if (blockIdx.x == 0) {
-        const int k = threadIdx.x;
-        if (k < p.nsoft && p.soft_bpos[k] < 0 && in_sig) {
-            const long long i = p.soft_idx[k];
-            C r = S::ld(R[i]);
-            if constexpr (S::kScaled) r = r * sRi;
-            r = r + (C)p.soft_sig[(long long)k * p.soft_len + n];
-            R[i] = store_round<T>(r * sR, p.sr, (unsigned long long)i, n, kSaltSource);
-        }
-        __syncthreads();
-        if (k < p.nrcv && p.rcv_bpos[k] < 0 && in_ir) {
-            C r = S::ld(R[p.rcv_idx[k]]);
-            if constexpr (S::kScaled) r = r * sRi;
-            p.ir[(long long)k * p.ir_cap + n] = (double)r;
-        }
-        // RMS rule of step n over receiver 0's samples [n-100, n), in float
-        // in the reference's order (simulator_gpu.cu:303-330)
-        StepCtl *ctl = p.ctl;
-        long long next = n + 1;
-        if (ctl->es_enable && p.nrcv > 0 && n > kRmsMinStep && n <= p.ir_cap) {
-            __shared__ float win[kRmsWindow];
-            if (k < kRmsWindow) win[k] = (float)p.ir[n - kRmsWindow + k];
-            __syncthreads();
-            if (k == 0) {
-                float sum = 0.0f;
-                for (int i = 0; i < kRmsWindow; ++i) sum += win[i] * win[i];
-                float rms = __fsqrt_rn(sum / (float)kRmsWindow);
-                if (n == kRmsMinStep + 1) {
-                    ctl->rms_state = rms;
-                } else {
-                    const float fg = 0.00008f;
-                    const float vout = (fg * rms + ctl->rms_state) / (fg + 1.f);
-                    ctl->rms_state = 2.f * vout - ctl->rms_state;
-                    rms = vout;
-                }
-                ctl->percent = ctl->es_target / rms;
-                if (rms < ctl->es_target) {
-                    ctl->stop_at = n;
-                    next |= kStopBit;
-                }
+    // Block 0: the points off the list, the RMS rule and the step counter.
+    // Every table load is issued at once.  While a soft source is active,
+    // its thread also writes the samples of the receivers on its cell, so
+    // no receiver reads a cell that another thread of the launch writes
+    // (and no barrier orders the two).
+    const int k = threadIdx.x;
+    StepCtl *ctl = p.ctl;
+    const bool rms = ctl->es_enable && p.nrcv > 0 && n > kRmsMinStep && n <= p.ir_cap;
+    __shared__ float win[kRmsWindow];
+    if (rms && k < kRmsWindow) win[k] = (float)p.ir[n - kRmsWindow + k];
+    if (k < p.nsoft && in_sig) {
+        const long long i = p.soft_idx[k], bp = p.soft_bpos[k];
+        const int j0 = p.src_rcv[k];
+        const C sg = (C)p.soft_sig[(long long)k * p.soft_len + n];
+        C r = S::ld(R[i]);
+        if constexpr (S::kScaled) r = r * sRi;
+        r = r + sg;
+        const T t = store_round<T>(r * sR, p.sr, (unsigned long long)i, n, kSaltSource);
+        if (bp < 0) {
+            R[i] = t;
+            if (in_ir) {
+                C v = S::ld(t);
+                if constexpr (S::kScaled) v = v * sRi;
+                for (int j = j0; j >= 0; j = p.rcv_next[j]) p.ir[(long long)j * p.ir_cap + n] = (double)v;
             }
         }
-        if (k == 0) ctl->nring[p.next_slot] = next;
     }
+    if (k < p.nrcv && in_ir) {
+        const long long i = p.rcv_idx[k], bp = p.rcv_bpos[k];
+        const bool by_source = in_sig && p.rcv_src[k] >= 0;
+        C r = S::ld(R[i]);
+        if constexpr (S::kScaled) r = r * sRi;
+        if (bp < 0 && !by_source) p.ir[(long long)k * p.ir_cap + n] = (double)r;
+    }
+    // RMS rule of step n over receiver 0's samples [n-100, n), in float in
+    // the reference's order (simulator_gpu.cu:303-330)
+    long long next = n + 1;
+    if (rms) {
+        __syncthreads();
+        if (k == 0) {
+            float sum = 0.0f;
+            for (int i = 0; i < kRmsWindow; ++i) sum += win[i] * win[i];
+            float rms_v = __fsqrt_rn(sum / (float)kRmsWindow);
+            if (n == kRmsMinStep + 1) {
+                ctl->rms_state = rms_v;
+            } else {
+                const float fg = 0.00008f;
+                const float vout = (fg * rms_v + ctl->rms_state) / (fg + 1.f);
+                ctl->rms_state = 2.f * vout - ctl->rms_state;
+                rms_v = vout;
+            }
+            ctl->percent = ctl->es_target / rms_v;
+            if (rms_v < ctl->es_target) {
+                ctl->stop_at = n;
+                next |= kStopBit;
+            }
+        }
+    }
+    if (k == 0) ctl->nring[p.next_slot] = next;
 }
 
 // Reverse boundary step.  With w = g*rbar the adjoint of the field after
@@ -1180,27 +1217,61 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
     T *__restrict__ R = (T *)p.R;
     const BoundaryState &bs = p.bs;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long long e = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (blockIdx.x == 0) {
+        // Block 0: dL/dIR into the adjoint field at the receivers off the
+        // list, and the step counter.  Receivers that share a cell are
+        // summed by the first of them, in index order, so no two threads
+        // update one cell.
+        const int k = threadIdx.x;
+        if (k < p.nrcv && in_g && p.rcv_bpos[k] < 0 && p.rcv_first[k]) {
+            const long long i = p.rcv_idx[k];
+            double g = 1.0;
+            sponge_g(p, i, g);
+            C acc = (C)((double)R[i] + g * p.gsig[(long long)k * p.ir_cap + n]);
+            for (int j = p.rcv_next[k]; j >= 0; j = p.rcv_next[j])
+                acc = (C)((double)acc + g * p.gsig[(long long)j * p.ir_cap + n]);
+            R[i] = acc;
+        }
+        if (k == 0) p.ctl->nring[p.next_slot] = n - 1;
+        return;
+    }
+    // list blocks (they exist only when the list is not empty)
+    const long long e = (long long)(blockIdx.x - 1) * kBlock + threadIdx.x;
     // this block's running sums, loaded now so the load is not the last link
     // of the block's chain (the sum below adds in the same order)
-    double *dst = p.grad_part + (long long)blockIdx.x * 5 * bs.nmat + threadIdx.x;
+    double *dst = p.grad_part + (long long)(blockIdx.x - 1) * 5 * bs.nmat + threadIdx.x;
     const double prev = threadIdx.x < 5 * bs.nmat ? *dst : 0.0;
     double gv[5] = {0, 0, 0, 0, 0};
     int m = -1;
-    if (e < bs.nb) {
-        const long long b = p.bidx[e];
-        m = bs.bmat ? bs.bmat[e] : 0;
-        const double *cf = bs.coeffs + 5 * m;
+    const long long nb = bs.nb;
+    {
+        // as in k_tail: every load that e names is issued at once from e
+        // clamped into the list, and the stores are predicated
+        const long long ec = e < nb ? e : nb - 1;
+        const uint32_t b = p.bidx[ec];
+        const int mm = bs.bmat[ec];
+        const int sp = (int)(n & 1);
+        const double xa = ((const double *)bs.xs[0])[ec], xb = ((const double *)bs.xs[1])[ec];
+        const double ya = ((const double *)bs.ys[0])[ec], yc = ((const double *)bs.ys[1])[ec];
+        const long long sl = n - bs.trace_base + 2;
+        const T *td = (const T *)bs.trace_d, *ty = (const T *)bs.trace_y;
+        const double dn = (double)td[sl * nb + ec];
+        const double x1 = (double)td[(sl - 1) * nb + ec];
+        const double x2 = (double)td[(sl - 2) * nb + ec];
+        const double y1 = (double)ty[(sl - 1) * nb + ec];
+        const double y2 = (double)ty[(sl - 2) * nb + ec];
+        const T rt = R[b];
+        const double *cf = bs.coeffs + 5 * mm;
         const bool spg = p.sp.w > 0;
         double g = 1.0;
-        double rb = (double)R[b];
+        double rb = (double)rt;
         if (spg) {
-            sponge_g(p, b, g);
+            sponge_g32(p, b, g);
             rb = rb / g;
         }
-        const int sp = (int)(n & 1);
-        const double *dbr0 = (const double *)bs.xs[sp ^ 1], *dbr1 = (const double *)bs.xs[sp];
-        const double yb1 = ((const double *)bs.ys[sp ^ 1])[e], yb2 = ((const double *)bs.ys[sp])[e], db1 = dbr0[e], db2 = dbr1[e];
+        // slot sp holds step n+2's adjoint taps, slot sp^1 step n+1's
+        const double yb1 = sp ? ya : yc, yb2 = sp ? yc : ya;
+        const double db1 = sp ? xa : xb, db2 = sp ? xb : xa;
         rb = rb + db1 - db2;
         if (in_g && p.list_points)
             for (int k = 0; k < p.nrcv; ++k)
@@ -1208,31 +1279,17 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
         const double kap = (double)bs.kconst * (double)0.4f;
         const double yb = -0.1 * kap * rb - cf[3] * yb1 - cf[4] * yb2;
         const double db = -0.9 * kap * rb + cf[0] * yb + cf[1] * yb1 + cf[2] * yb2;
-        const long long sl = n - bs.trace_base + 2;
-        const T *td = (const T *)bs.trace_d, *ty = (const T *)bs.trace_y;
-        const double dn = (double)td[sl * bs.nb + e];
-        const double x1 = (double)td[(sl - 1) * bs.nb + e];
-        const double x2 = (double)td[(sl - 2) * bs.nb + e];
-        const double y1 = (double)ty[(sl - 1) * bs.nb + e];
-        const double y2 = (double)ty[(sl - 2) * bs.nb + e];
-        gv[0] = yb * dn;
-        gv[1] = yb * x1;
-        gv[2] = yb * x2;
-        gv[3] = -yb * y1;
-        gv[4] = -yb * y2;
-        ((double *)bs.ys[sp])[e] = yb;
-        ((double *)bs.xs[sp])[e] = db;
-        R[b] = (C)(spg ? rb * g : rb);
-    }
-    if (blockIdx.x == 0) {
-        const int k = threadIdx.x;
-        if (k < p.nrcv && p.rcv_bpos[k] < 0 && in_g) {
-            const long long i = p.rcv_idx[k];
-            double g = 1.0;
-            sponge_g(p, i, g);
-            R[i] = (C)((double)R[i] + g * p.gsig[(long long)k * p.ir_cap + n]);
+        if (e < nb) {
+            m = mm;
+            gv[0] = yb * dn;
+            gv[1] = yb * x1;
+            gv[2] = yb * x2;
+            gv[3] = -yb * y1;
+            gv[4] = -yb * y2;
+            ((double *)bs.ys[sp])[e] = yb;
+            ((double *)bs.xs[sp])[e] = db;
+            R[b] = (C)(spg ? rb * g : rb);
         }
-        if (k == 0) p.ctl->nring[p.next_slot] = n - 1;
     }
     // per-material block sums, fixed reduction order.  Lanes without an
     // entry hold m = -1 and zero terms; a wave whose entries share one

@@ -73,6 +73,8 @@ struct virs_ctx {
 
     int nrcv = 0;
     long long *rcv_idx = nullptr, *rcv_bpos = nullptr;
+    // TailParams::rcv_src/rcv_next/rcv_first/src_rcv, [4][kMaxPoints]
+    int *rcv_aux = nullptr;
     bool points_on_list = false; // some soft source or receiver is a list cell
     std::vector<long long> h_rcv_idx;
     double *ir_buf = nullptr;
@@ -205,6 +207,22 @@ int upload_points(virs_ctx *c)
         HIPC(c, hipMemcpy(c->rcv_idx, c->h_rcv_idx.data(), sizeof(long long) * c->nrcv, hipMemcpyHostToDevice));
         HIPC(c, hipMemcpy(c->rcv_bpos, rb.data(), sizeof(long long) * c->nrcv, hipMemcpyHostToDevice));
     }
+    // the soft source on each receiver's cell, the receivers that share a
+    // cell as chains in index order, and each source's first receiver
+    std::vector<int> aux(4 * (size_t)kMaxPoints, -1);
+    for (int k = 0; k < c->nrcv; ++k) {
+        for (int j = 0; j < c->nsoft; ++j)
+            if (c->h_soft_idx[j] == c->h_rcv_idx[k]) aux[k] = j;
+        for (int j = k + 1; j < c->nrcv && aux[kMaxPoints + k] < 0; ++j)
+            if (c->h_rcv_idx[j] == c->h_rcv_idx[k]) aux[kMaxPoints + k] = j;
+        aux[2 * kMaxPoints + k] = 1;
+        for (int j = 0; j < k; ++j)
+            if (c->h_rcv_idx[j] == c->h_rcv_idx[k]) aux[2 * kMaxPoints + k] = 0;
+    }
+    for (int j = 0; j < c->nsoft; ++j)
+        for (int k = c->nrcv - 1; k >= 0; --k)
+            if (c->h_rcv_idx[k] == c->h_soft_idx[j]) aux[3 * kMaxPoints + j] = k;
+    HIPC(c, hipMemcpy(c->rcv_aux, aux.data(), sizeof(int) * aux.size(), hipMemcpyHostToDevice));
     return VIRS_OK;
 }
 
@@ -274,6 +292,10 @@ TailParams tail_params(const virs_ctx *c, const void *P, const void *Q, void *R,
     t.nrcv = c->nrcv;
     t.rcv_idx = c->rcv_idx;
     t.rcv_bpos = c->rcv_bpos;
+    t.rcv_src = c->rcv_aux;
+    t.rcv_next = c->rcv_aux + kMaxPoints;
+    t.rcv_first = c->rcv_aux + 2 * kMaxPoints;
+    t.src_rcv = c->rcv_aux + 3 * kMaxPoints;
     t.list_points = c->points_on_list ? 1 : 0;
     t.ir = c->ir_buf;
     t.ir_cap = c->ir_cap;
@@ -287,7 +309,10 @@ TailParams tail_params(const virs_ctx *c, const void *P, const void *Q, void *R,
     return t;
 }
 
-int tail_grid(long long nb) { return (int)std::max<long long>(1, (nb + kBlock - 1) / kBlock); }
+// List launches: block 0 takes the points and the step counter, blocks
+// 1.. one list entry per thread.
+long long list_blocks(long long nb) { return (nb + kBlock - 1) / kBlock; }
+int tail_grid(long long nb) { return (int)(1 + list_blocks(nb)); }
 
 // Stencil grid.  capacity = resident blocks of the whole GPU: occupancy x
 // CUs, the occupancy capped at VIRS_DENSE_OCC_CAP blocks (12 waves) per CU.
@@ -658,6 +683,7 @@ int virs_create(const virs_desc *d, virs_ctx **out)
     if (!rc) rc = dalloc(c, &c->soft_bpos, sizeof(long long) * kMaxPoints);
     if (!rc) rc = dalloc(c, &c->rcv_idx, sizeof(long long) * kMaxPoints);
     if (!rc) rc = dalloc(c, &c->rcv_bpos, sizeof(long long) * kMaxPoints);
+    if (!rc) rc = dalloc(c, &c->rcv_aux, sizeof(int) * 4 * kMaxPoints);
     if (!rc) rc = dalloc(c, &c->neg, sizeof(long long) * kMaxPoints);
     if (!rc && hipMemset(c->neg, 0xff, sizeof(long long) * kMaxPoints) != hipSuccess) rc = VIRS_ERR_HIP;
     if (!rc) rc = reset_run(c);
@@ -705,6 +731,7 @@ int virs_destroy(virs_ctx *c)
     dfree(c->soft_sig);
     dfree(c->rcv_idx);
     dfree(c->rcv_bpos);
+    dfree(c->rcv_aux);
     dfree(c->neg);
     dfree(c->bstart);
     dfree(c->ir_buf);
@@ -1610,7 +1637,7 @@ int virs_adjoint(virs_ctx *c, const float *dl_dir, double *grad)
         for (long long i = 0; i < (long long)c->nrcv * T; ++i) g[i] = dl_dir[i];
         HIPC(c, hipMemcpy(c->gsig, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice));
     }
-    const long long gb = tail_grid(c->nb);
+    const long long gb = std::max<long long>(1, list_blocks(c->nb)); // one row per list block
     if (c->grad_blocks != gb || !c->grad_part) {
         dfree(c->grad_part);
         rc = dalloc(c, &c->grad_part, sizeof(double) * 5 * 8 * gb);
