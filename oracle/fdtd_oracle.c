@@ -392,7 +392,8 @@ int oracle_adjoint_f64(const oracle_desc *d, int64_t nsteps, const double *dL_di
  *   - walls and border cells hold zero, so the stencil sums the six
  *     neighbours as stored and weights the centre by 6 - (wall neighbours),
  *     which is the reference's substitution V(wall) = pc in exact arithmetic,
- *     and multiplies by the one factor c2dt2*ih2 (the FP32 rounding differs
+ *     and multiplies by the one factor c2dt2*ih2; with stochastic rounding
+ *     the last two steps are fused multiply-adds (the FP32 rounding differs
  *     from oracle_run_f32's);
  *   - the boundary list runs after the stencil on the stored field, soft
  *     sources after the boundary correction on its stored result.
@@ -590,13 +591,21 @@ int oracle_run_s16(const oracle_desc *d, int bf16, int sr, int64_t nsteps, doubl
                         const int walls = (d->flags[i + 1] == 1) + (d->flags[i - 1] == 1) +
                                           (d->flags[i + nx] == 1) + (d->flags[i - nx] == 1) +
                                           (d->flags[i + sxy] == 1) + (d->flags[i - sxy] == 1);
-                        const float lapc = (sum - (float)(6 - walls) * pc) * (c2dt2 * ih2);
+                        /* t = sum - kw*pc, u = a*pc - b*q, R = t*(g*c2dt2*ih2) + u;
+                         * with stochastic rounding u and R are fused
+                         * multiply-adds (each rounds once), with nearest
+                         * rounding every product rounds apart */
+                        const float t = sum - (float)(6 - walls) * pc;
                         float a = 2.f, b = 1.f, g = 1.f;
                         if (d->sponge_w > 0) {
                             const int dd = sponge_dist(d, x, y, z);
                             if (dd < d->sponge_w) { a = d->spg_a[dd]; b = d->spg_b[dd]; g = d->spg_g[dd]; }
                         }
-                        o = (a * pc - b * (ld16(&S, Q[i]) * sQ)) + g * lapc;
+                        const float qq = ld16(&S, Q[i]) * sQ;
+                        if (sr)
+                            o = fmaf(t, g * (c2dt2 * ih2), fmaf(a, pc, -(b * qq)));
+                        else
+                            o = (a * pc - b * qq) + g * (t * (c2dt2 * ih2));
                     }
                     if (fabsf(o) > mx) mx = fabsf(o);
                     R[i] = st16(&S, o * sR, (uint64_t)i, n, SALT_DENSE);

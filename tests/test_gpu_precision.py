@@ -5,8 +5,8 @@
    oracle's restatement of the same storage rules, on small grids.
 2. Accuracy against the FP64 run of the same scheme on the 5x4x3 m shoebox
    with 8 receivers (2000 steps); the largest relative L2 error over the
-   receivers, measured: FP32 1.9e-5; FP16 4.8e-3 with stochastic rounding,
-   5.0e-3 with nearest-even rounding; BF16 3.3e-2 (stochastic).  The FP16
+   receivers, measured: FP32 1.9e-5; FP16 4.5e-3 with stochastic rounding,
+   5.0e-3 with nearest-even rounding; BF16 3.4e-2 (stochastic).  The FP16
    error is a noise floor of the 11-bit significand: the per-store rounding
    noise accumulates over the room's decay time.  The 1e-3 goal for FP16 is
    not met; the bounds below pin the measured levels.

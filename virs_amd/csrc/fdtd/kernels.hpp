@@ -778,7 +778,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                     sum = sum + PP[j];
                     sum = sum + PM[j];
                     const C kw = (C)(((j & 1) ? kwo : kwe) >> (8 * (j >> 1)) & 0xffu); // cells that update
-                    const C lapc = (sum - kw * pcv) * c2ih2;
+                    // t = sum - kw*pc, u = a*pc - b*q, R = t*(g*c2dt2*ih2) + u.
+                    // With stochastic rounding u and R are fused multiply-adds
+                    // (each rounds once).  Nearest rounding keeps every product
+                    // rounded apart: its rounding errors do not average out, and
+                    // the fused final step drove the room's constant mode
+                    // (relative IR error 0.36 against 5e-3 on the 2000-step
+                    // shoebox).
+                    const C t = sum - kw * pcv;
                     C a = (C)2, b = (C)1, g = (C)1;
                     if constexpr (SPONGE) {
                         const int d = edge_dist(x0 + j, y, z, nx, ny, nz);
@@ -788,7 +795,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                             g = (C)sp_tab[2][d];
                         }
                     }
-                    const C upd = (a * pcv - b * qv[j]) + g * lapc;
+                    C upd;
+                    if constexpr (SR) {
+                        if constexpr (SPONGE)
+                            upd = __builtin_fmaf(t, g * c2ih2, __builtin_fmaf(a, pcv, -(b * qv[j])));
+                        else
+                            upd = __builtin_fmaf(t, c2ih2, __builtin_fmaf((C)2, pcv, -qv[j]));
+                    } else {
+                        upd = (a * pcv - b * qv[j]) + g * (t * c2ih2);
+                    }
                     o[j] = bsel(bit_mask(c, 3), bsel(bit_mask(c, 0), hard_u, (C)0), upd);
                     continue;
                 }
@@ -1052,13 +1067,21 @@ __device__ inline void sponge_g32(const TailParams &p, uint32_t idx, C &g)
     }
 }
 
+// List entries per thread of the forward list launch.  One is fastest for
+// every precision on the 512x384x256 room (FP16 12.3 / 13.5 / 14.1 us and
+// FP32 15.8 / 16.8 / 18.3 us at 1 / 2 / 4 entries).
+#ifndef VIRS_TAIL_ENTRIES
+#define VIRS_TAIL_ENTRIES 1
+#endif
+constexpr int kTailEntries = VIRS_TAIL_ENTRIES;
+
 // The rest of a step after k_dense: the boundary biquad over the compact
 // list when it was not fused into the stencil, then soft sources (added to
 // the corrected field), receiver gather, and the step counter.  Point cells
 // on the list are handled by their entry's thread, the others by block 0.
 //
-// One entry per thread (two or four per thread were slower).  The launch is
-// bound by latency, not bytes, so every load that the entry index e names
+// One entry per thread (kTailEntries).  The launch is bound by latency more
+// than bytes, so every load that the entry index e names
 // (the list index, the material byte, both slots of the pressure ring hv and
 // of the filter taps) is issued at once, from e clamped into the list, and
 // none depends on the step counter: both ring slots are read every step, and
@@ -1092,48 +1115,69 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
     const bool in_ir = n >= 0 && n < p.ir_cap;
     const long long nb = p.bs.nb;
     if (blockIdx.x > 0) { // list blocks exist only when nb > 0
-        const long long e = (long long)(blockIdx.x - 1) * kBlock + threadIdx.x;
-        const long long ec = e < nb ? e : nb - 1;
-        const uint32_t b = p.bidx[ec];
-        const uint8_t m = p.bs.bmat[ec]; // always present for the list launch
-        const T h0 = ((const T *)p.hv[0])[ec], h1 = ((const T *)p.hv[1])[ec];
-        const C xa = ((const C *)p.bs.xs[0])[ec], xb = ((const C *)p.bs.xs[1])[ec];
-        const Y ya = ((const Y *)p.bs.ys[0])[ec], yb = ((const Y *)p.bs.ys[1])[ec];
-        const T rt = R[b];
-        const double *cf = p.bs.coeffs + 5 * m;
+        constexpr int U = kTailEntries;
+        // block b covers entries [(b-1)*U*kBlock, b*U*kBlock), lane-contiguous
+        // per slot u; all loads of all U entries go out before the gathers
+        long long ev[U];
+        uint32_t bv[U];
+        uint8_t mv[U];
+        T h0[U], h1[U], rt[U];
+        C xa[U], xb[U];
+        Y ya[U], yb[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ev[u] = ((long long)(blockIdx.x - 1) * U + u) * kBlock + threadIdx.x;
+            const long long ec = ev[u] < nb ? ev[u] : nb - 1;
+            bv[u] = p.bidx[ec];
+            mv[u] = p.bs.bmat[ec]; // always present for the list launch
+            h0[u] = ((const T *)p.hv[0])[ec];
+            h1[u] = ((const T *)p.hv[1])[ec];
+            xa[u] = ((const C *)p.bs.xs[0])[ec];
+            xb[u] = ((const C *)p.bs.xs[1])[ec];
+            ya[u] = ((const Y *)p.bs.ys[0])[ec];
+            yb[u] = ((const Y *)p.bs.ys[1])[ec];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) rt[u] = R[bv[u]];
         // before step n, slot n&1 holds step n-2's value and slot (n+1)&1
         // step n-1's (the ring rule of BoundaryState and hv)
         const bool odd = (n & 1) != 0;
-        C pp = S::ld(odd ? h1 : h0), qq = S::ld(odd ? h0 : h1), r = S::ld(rt);
-        const double x1 = (double)(odd ? xa : xb), x2 = (double)(odd ? xb : xa);
-        const double y1 = (double)(odd ? ya : yb), y2 = (double)(odd ? yb : ya);
-        if constexpr (S::kScaled) pp = pp * sP, qq = qq * sQ, r = r * sRi;
-        const C dif = pp - qq;
-        const double in = (double)dif;
-        const double yv = biquad_out(cf, in, x1, x2, y1, y2);
-        r = corrected<C>(r, p.bs.kconst, yv, in);
-        T t = store_round<T>(r * sR, p.sr, (unsigned long long)b, n, kSaltBoundary);
-        if (e < nb && !skip) {
-            store_taps<C, Y>(p.bs, e, n, dif, yv);
-            // a soft source on this cell is added to the stored corrected
-            // value (the point tables are scanned only when some point is on
-            // the list)
-            for (int k = 0; k < (p.list_points ? p.nsoft : 0); ++k)
-                if (p.soft_bpos[k] == e && in_sig) {
-                    C r2 = S::ld(t);
-                    if constexpr (S::kScaled) r2 = r2 * sRi;
-                    r2 = r2 + (C)p.soft_sig[(long long)k * p.soft_len + n];
-                    t = store_round<T>(r2 * sR, p.sr, (unsigned long long)b, n, kSaltSource);
-                }
-            R[b] = t;
-            ((T *)p.hv[(n + 1) & 1])[e] = t;
-            if (in_ir && p.list_points)
-                for (int k = 0; k < p.nrcv; ++k)
-                    if (p.rcv_bpos[k] == e) {
-                        C stored = S::ld(t);
-                        if constexpr (S::kScaled) stored = stored * sRi;
-                        p.ir[(long long)k * p.ir_cap + n] = (double)stored;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long e = ev[u];
+            const uint32_t b = bv[u];
+            const double *cf = p.bs.coeffs + 5 * mv[u];
+            C pp = S::ld(odd ? h1[u] : h0[u]), qq = S::ld(odd ? h0[u] : h1[u]), r = S::ld(rt[u]);
+            const double x1 = (double)(odd ? xa[u] : xb[u]), x2 = (double)(odd ? xb[u] : xa[u]);
+            const double y1 = (double)(odd ? ya[u] : yb[u]), y2 = (double)(odd ? yb[u] : ya[u]);
+            if constexpr (S::kScaled) pp = pp * sP, qq = qq * sQ, r = r * sRi;
+            const C dif = pp - qq;
+            const double in = (double)dif;
+            const double yv = biquad_out(cf, in, x1, x2, y1, y2);
+            r = corrected<C>(r, p.bs.kconst, yv, in);
+            T t = store_round<T>(r * sR, p.sr, (unsigned long long)b, n, kSaltBoundary);
+            if (e < nb && !skip) {
+                store_taps<C, Y>(p.bs, e, n, dif, yv);
+                // a soft source on this cell is added to the stored corrected
+                // value (the point tables are scanned only when some point is
+                // on the list)
+                for (int k = 0; k < (p.list_points ? p.nsoft : 0); ++k)
+                    if (p.soft_bpos[k] == e && in_sig) {
+                        C r2 = S::ld(t);
+                        if constexpr (S::kScaled) r2 = r2 * sRi;
+                        r2 = r2 + (C)p.soft_sig[(long long)k * p.soft_len + n];
+                        t = store_round<T>(r2 * sR, p.sr, (unsigned long long)b, n, kSaltSource);
                     }
+                R[b] = t;
+                ((T *)p.hv[(n + 1) & 1])[e] = t;
+                if (in_ir && p.list_points)
+                    for (int k = 0; k < p.nrcv; ++k)
+                        if (p.rcv_bpos[k] == e) {
+                            C stored = S::ld(t);
+                            if constexpr (S::kScaled) stored = stored * sRi;
+                            p.ir[(long long)k * p.ir_cap + n] = (double)stored;
+                        }
+            }
         }
         return;
     }
