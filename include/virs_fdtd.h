@@ -105,7 +105,9 @@ int virs_set_materials(virs_ctx *ctx, const double *coeffs, int32_t nmat);
 int virs_set_hard_source(virs_ctx *ctx, const double *sig, int64_t len);
 /* Soft (additive) point sources, no reference counterpart: after the
  * stencil and the boundary correction, p[idx[k]] += sig[k][n].  Cells must
- * be distinct. */
+ * be distinct and must update (air, boundary shell or flag 3): a source on a
+ * wall or on the domain border is rejected, here or by a later
+ * virs_upload_flags. */
 int virs_set_soft_sources(virs_ctx *ctx, const int64_t *idx, const double *sig, int32_t nsrc,
                           int64_t len);
 /* Receiver voxels.  The reference has one, fixed at (334,225,270)

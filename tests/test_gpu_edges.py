@@ -81,3 +81,31 @@ def test_points_at_their_limits(gpu, precision):
     np.testing.assert_array_equal(ir[60:64], ir[40:44])  # shared cells read alike
     np.testing.assert_array_equal(f.field(0), ref["p"])
     f.close()
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp16"])
+def test_soft_source_on_a_wall_or_border_is_rejected(gpu, precision):
+    """A soft source must sit on a cell that updates: one on a wall or on
+    the domain border is rejected when the sources are set after the flags,
+    and by the flag upload when they were set before."""
+    from virs_amd.fdtd import FDTD, FDTDError
+    p = tiny_problem(3)
+    flags = p["flags"]
+    nz, ny, nx = flags.shape
+    wall = int(np.flatnonzero(flags.reshape(-1) == 1)[0])
+    border = 0
+    air = int(p["soft_idx"][0])
+    sig = np.ones((1, 10))
+    _say(f"soft source on a wall {precision}")
+    f = FDTD((nx, ny, nz), p["h"], dt=p["dt"], precision=precision)
+    f.upload_flags(flags)
+    for cell in (wall, border):
+        with pytest.raises(FDTDError, match="wall or border"):
+            f.set_soft_sources([cell], sig)
+    f.set_soft_sources([air], sig)
+    f.close()
+    g = FDTD((nx, ny, nz), p["h"], dt=p["dt"], precision=precision)
+    g.set_soft_sources([wall], sig)
+    with pytest.raises(FDTDError, match="wall or border"):
+        g.upload_flags(flags)
+    g.close()

@@ -33,10 +33,12 @@ def max_edc_dev(ir, ref, floor_db=-60.0):
 
 @pytest.mark.parametrize("fmt", ["fp16", "bf16"])
 @pytest.mark.parametrize("rounding", ["nearest", "stochastic"])
-@pytest.mark.parametrize("seed", [0, 1])
-def test_16bit_storage_bit_exact(gpu, fmt, rounding, seed):
-    p = tiny_problem(30 + seed, steps=70)
-    print(f"16-bit {fmt} {rounding} seed={seed}", flush=True)
+@pytest.mark.parametrize("seed,hard", [(0, True), (1, False)])
+def test_16bit_storage_bit_exact(gpu, fmt, rounding, seed, hard):
+    """With a flag-3 cell the stencil selects the hard value; without one
+    the zero cells take the update with a zero Laplacian factor."""
+    p = tiny_problem(30 + seed, steps=70, hard=hard)
+    print(f"16-bit {fmt} {rounding} seed={seed} hard={hard}", flush=True)
     f = gpu_context(p, fmt, rounding=rounding)
     f.step(p["steps"])
     f.synchronize()

@@ -554,7 +554,10 @@ __device__ inline C boundary_update(const BoundaryState &bs, long long e, long l
 // FUSED: the boundary biquad of the block's list entries runs at the end of
 // the same launch.  ADJ: the reverse sweep applies the same operator to the adjoint
 // field; hard-source cells hold prescribed values, so their adjoint is zero.
-template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE, bool SR>
+// HARD (16-bit storage): the room has flag-3 cells.  Without them a wall or
+// border cell (P = Q = 0 there) gets the update with a zero Laplacian
+// factor, which is exactly +0, in place of two selects per cell.
+template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE, bool SR, bool HARD = true>
 #ifndef VIRS_DENSE_WAVES
 #define VIRS_DENSE_WAVES 1 // minimum waves per SIMD the register allocation must allow
 #endif
@@ -795,16 +798,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                             g = (C)sp_tab[2][d];
                         }
                     }
+                    // Laplacian factor: zero on zero cells when the room has
+                    // no hard cells (a*0 - b*0 and t*0 then give +0)
+                    const C kf = HARD ? c2ih2 : bsel(bit_mask(c, 3), (C)0, c2ih2);
                     C upd;
                     if constexpr (SR) {
                         if constexpr (SPONGE)
-                            upd = __builtin_fmaf(t, g * c2ih2, __builtin_fmaf(a, pcv, -(b * qv[j])));
+                            upd = __builtin_fmaf(t, g * kf, __builtin_fmaf(a, pcv, -(b * qv[j])));
                         else
-                            upd = __builtin_fmaf(t, c2ih2, __builtin_fmaf((C)2, pcv, -qv[j]));
+                            upd = __builtin_fmaf(t, kf, __builtin_fmaf((C)2, pcv, -qv[j]));
                     } else {
-                        upd = (a * pcv - b * qv[j]) + g * (t * c2ih2);
+                        upd = (a * pcv - b * qv[j]) + g * (t * kf);
                     }
-                    o[j] = bsel(bit_mask(c, 3), bsel(bit_mask(c, 0), hard_u, (C)0), upd);
+                    o[j] = HARD ? bsel(bit_mask(c, 3), bsel(bit_mask(c, 0), hard_u, (C)0), upd) : upd;
                     continue;
                 }
                 const C xp = bsel(bit_mask(c, 0), pcv, j + 1 < V ? PC[(j + 1) % V] : right);

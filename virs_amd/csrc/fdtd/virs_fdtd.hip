@@ -37,6 +37,7 @@ struct virs_ctx {
     uint8_t *code = nullptr;
     uint8_t *normals = nullptr;
     bool have_flags = false;
+    long long n_hard = 0;          // interior flag-3 cells (hard source)
     std::vector<uint32_t> h_flag2; // interior flag-2 cells of the uploaded flags
     int nchunks_plane = 0;         // 256-thread chunks per z-plane of the stencil
     long long *bstart = nullptr;   // [nz*nchunks_plane + 1] list offsets per (chunk, z)
@@ -348,10 +349,10 @@ int dense_grid(const virs_ctx *c, K kernel, int &zranges)
     return (int)std::max<long long>(1, std::min<long long>(g, W));
 }
 
-template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE, bool SR>
+template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE, bool SR, bool HARD = true>
 hipError_t launch_dense_k(const virs_ctx *c, const DenseParams &p)
 {
-    auto k = k_dense<T, V, ADJ, FUSED, SPONGE, SR>;
+    auto k = k_dense<T, V, ADJ, FUSED, SPONGE, SR, HARD>;
     DenseParams q = p;
     const int grid = dense_grid(c, k, q.zranges);
     if (c->kev[0])
@@ -359,6 +360,23 @@ hipError_t launch_dense_k(const virs_ctx *c, const DenseParams &p)
     else
         hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, c->stream, q);
     return hipGetLastError();
+}
+
+template <typename T, int V, bool HARD>
+hipError_t launch_dense_s16(const virs_ctx *c, const DenseParams &p, bool sponge)
+{
+    if (p.sr) {
+        if (p.bstart)
+            return sponge ? launch_dense_k<T, V, false, true, true, true, HARD>(c, p)
+                          : launch_dense_k<T, V, false, true, false, true, HARD>(c, p);
+        return sponge ? launch_dense_k<T, V, false, false, true, true, HARD>(c, p)
+                      : launch_dense_k<T, V, false, false, false, true, HARD>(c, p);
+    }
+    if (p.bstart)
+        return sponge ? launch_dense_k<T, V, false, true, true, false, HARD>(c, p)
+                      : launch_dense_k<T, V, false, true, false, false, HARD>(c, p);
+    return sponge ? launch_dense_k<T, V, false, false, true, false, HARD>(c, p)
+                  : launch_dense_k<T, V, false, false, false, false, HARD>(c, p);
 }
 
 template <typename T, int V, bool ADJ>
@@ -369,19 +387,10 @@ hipError_t launch_dense_v(const virs_ctx *c, const DenseParams &p)
     } else {
         const bool sponge = p.sp.w > 0;
         if constexpr (Store<T>::kScaled) {
-            // stochastic or nearest storage rounding, chosen at compile time
-            if (p.sr) {
-                if (p.bstart)
-                    return sponge ? launch_dense_k<T, V, false, true, true, true>(c, p)
-                                  : launch_dense_k<T, V, false, true, false, true>(c, p);
-                return sponge ? launch_dense_k<T, V, false, false, true, true>(c, p)
-                              : launch_dense_k<T, V, false, false, false, true>(c, p);
-            }
-            if (p.bstart)
-                return sponge ? launch_dense_k<T, V, false, true, true, false>(c, p)
-                              : launch_dense_k<T, V, false, true, false, false>(c, p);
-            return sponge ? launch_dense_k<T, V, false, false, true, false>(c, p)
-                          : launch_dense_k<T, V, false, false, false, false>(c, p);
+            // stochastic or nearest storage rounding and the presence of
+            // hard cells, chosen at compile time
+            if (c->n_hard > 0) return launch_dense_s16<T, V, true>(c, p, sponge);
+            return launch_dense_s16<T, V, false>(c, p, sponge);
         } else {
             if constexpr (!ADJ) {
                 if (p.bstart) {
@@ -768,6 +777,24 @@ int virs_get_constants(const virs_ctx *c, float *dt, float *c2dt2, float *ih2, f
     return VIRS_OK;
 }
 
+// Soft sources must sit on cells that update (air, boundary shell or a hard
+// cell), not on walls or the domain border: those hold zero in 16-bit
+// storage and are never read by FP32/FP64 neighbours.  Reads the cell codes
+// of the (at most kMaxPoints) source cells back from the device.
+int check_soft_cells(virs_ctx *c, const long long *idx, int nsrc)
+{
+    if (!c->have_flags) return VIRS_OK;
+    const bool s16 = c->d.precision == VIRS_FP16 || c->d.precision == VIRS_BF16;
+    for (int k = 0; k < nsrc; ++k) {
+        const long long i = idx[k];
+        uint8_t b = 0;
+        HIPC(c, hipMemcpy(&b, c->code + (s16 ? i >> 1 : i), 1, hipMemcpyDeviceToHost));
+        const bool zero = s16 ? ((b >> (4 * (i & 1))) & 0xFu) == kNib16Zero : ((b >> 6) & 3u) == 1u;
+        if (zero) return fail(c, VIRS_ERR_ARG, "source %d (cell %lld) sits on a wall or border cell", k, i);
+    }
+    return VIRS_OK;
+}
+
 int virs_upload_flags(virs_ctx *c, const uint8_t *flags)
 {
     if (!c || !flags) return VIRS_ERR_ARG;
@@ -801,13 +828,17 @@ int virs_upload_flags(virs_ctx *c, const uint8_t *flags)
     if (e != hipSuccess) return fail(c, VIRS_ERR_HIP, "flag upload: %s", hipGetErrorString(e));
     c->have_flags = true;
     c->h_flag2.clear();
+    c->n_hard = 0;
     const long long nx = c->nx, ny = c->ny, nz = c->nz;
     for (long long z = 1; z < nz - 1; ++z)
         for (long long y = 1; y < ny - 1; ++y)
             for (long long x = 1; x < nx - 1; ++x) {
                 const long long i = (z * ny + y) * nx + x;
                 if (flags[i] == 2) c->h_flag2.push_back((uint32_t)i);
+                c->n_hard += flags[i] == 3;
             }
+    rc = check_soft_cells(c, c->h_soft_idx.data(), c->nsoft);
+    if (rc) return rc;
     drop_graphs(c);
     return refresh_fusion(c);
 }
@@ -907,6 +938,11 @@ int virs_set_soft_sources(virs_ctx *c, const int64_t *idx, const double *sig, in
     }
     HIPC(c, hipSetDevice(c->d.device));
     HIPC(c, hipStreamSynchronize(c->stream));
+    {
+        std::vector<long long> cells(idx, idx + nsrc);
+        int rc = check_soft_cells(c, cells.data(), nsrc);
+        if (rc) return rc;
+    }
     dfree(c->soft_sig);
     c->nsoft = nsrc;
     c->soft_len = nsrc ? len : 0;
