@@ -1056,21 +1056,17 @@ __device__ inline void sponge_g(const TailParams &p, long long idx, C &g)
     }
 }
 
-// sponge_g for a list cell: list indices are 32-bit, so the coordinates
-// come from 32-bit divisions (exact, and far shorter than 64-bit ones).
-template <typename C>
-__device__ inline void sponge_g32(const TailParams &p, uint32_t idx, C &g)
+// Distance of a list cell from the domain edge (the sponge table index):
+// list indices are 32-bit, so the coordinates come from 32-bit divisions
+// (exact, and far shorter than 64-bit ones).
+__device__ inline int sponge_dist32(const TailParams &p, uint32_t idx)
 {
-    g = (C)1;
-    if (p.sp.w > 0) {
-        const uint32_t nx = (uint32_t)p.nx, sxy = nx * (uint32_t)p.ny;
-        const uint32_t z = idx / sxy;
-        const uint32_t r = idx - z * sxy;
-        const uint32_t y = r / nx;
-        const uint32_t x = r - y * nx;
-        const int d = edge_dist((int)x, (int)y, (int)z, p.nx, p.ny, p.nz);
-        if (d < p.sp.w) g = (C)p.sp.g[d];
-    }
+    const uint32_t nx = (uint32_t)p.nx, sxy = nx * (uint32_t)p.ny;
+    const uint32_t z = idx / sxy;
+    const uint32_t r = idx - z * sxy;
+    const uint32_t y = r / nx;
+    const uint32_t x = r - y * nx;
+    return edge_dist((int)x, (int)y, (int)z, p.nx, p.ny, p.nz);
 }
 
 // List entries per thread of the forward list launch.  One is fastest for
@@ -1255,13 +1251,14 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
 //   dbar   = -0.9 kap rbar + b0 ybar + b1 ybar1 + b2 ybar2
 //   dL/d(b0,b1,b2,a1,a2) += ybar * (d^n, d^{n-1}, d^{n-2}, -y^{n-1}, -y^{n-2})
 // where d and y come from the forward trace of this segment.  The state
-// rings of BoundaryState hold ybar (ys) and dbar (xs).  Per-block gradient partials are
-// summed in a fixed order, so the result is deterministic.
+// rings of BoundaryState hold ybar (ys) and dbar (xs).  Per-wave gradient
+// partials are summed in a fixed order, so the result is deterministic.  (A
+// per-block sum through LDS put a barrier, which waits for the waves'
+// stores, at the end of every block: 17.0 against 10.0 us per launch.)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
 {
     using C = T;
-    __shared__ double part[kWaves][8 * 5];
     const long long n = p.ctl->nring[p.slot];
     const bool in_g = n >= 0 && n < p.ir_cap;
     T *__restrict__ R = (T *)p.R;
@@ -1287,10 +1284,15 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
     }
     // list blocks (they exist only when the list is not empty)
     const long long e = (long long)(blockIdx.x - 1) * kBlock + threadIdx.x;
-    // this block's running sums, loaded now so the load is not the last link
-    // of the block's chain (the sum below adds in the same order)
-    double *dst = p.grad_part + (long long)(blockIdx.x - 1) * 5 * bs.nmat + threadIdx.x;
-    const double prev = threadIdx.x < 5 * bs.nmat ? *dst : 0.0;
+    // this wave's running sums (one row per wave, so no barrier joins the
+    // waves), loaded now so the load is not the last link of the chain.
+    // With at most 16 sums (tree reduction below) lane 4s writes sum s,
+    // otherwise lane s.
+    const bool tree = 5 * bs.nmat <= 16;
+    const int pslot = tree ? ((lane & 3) == 0 ? lane >> 2 : 64) : lane;
+    const bool writer = pslot < 5 * bs.nmat;
+    double *dst = p.grad_part + ((long long)(blockIdx.x - 1) * kWaves + wave) * 5 * bs.nmat + (writer ? pslot : 0);
+    const double prev = writer ? *dst : 0.0;
     double gv[5] = {0, 0, 0, 0, 0};
     int m = -1;
     const long long nb = bs.nb;
@@ -1305,20 +1307,24 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
         const double ya = ((const double *)bs.ys[0])[ec], yc = ((const double *)bs.ys[1])[ec];
         const long long sl = n - bs.trace_base + 2;
         const T *td = (const T *)bs.trace_d, *ty = (const T *)bs.trace_y;
-        const double dn = (double)td[sl * nb + ec];
-        const double x1 = (double)td[(sl - 1) * nb + ec];
-        const double x2 = (double)td[(sl - 2) * nb + ec];
-        const double y1 = (double)ty[(sl - 1) * nb + ec];
-        const double y2 = (double)ty[(sl - 2) * nb + ec];
+        const T tdn = td[sl * nb + ec], tx1 = td[(sl - 1) * nb + ec], tx2 = td[(sl - 2) * nb + ec];
+        const T ty1 = ty[(sl - 1) * nb + ec], ty2 = ty[(sl - 2) * nb + ec];
         const T rt = R[b];
+        // sponge factor without a branch: the table entry is read for every
+        // cell (clamped into the table) and used where the cell lies in the
+        // sponge; x/1 and x*1 are exact, so rooms without a sponge get the
+        // same bits as skipping both
+        const int sd = sponge_dist32(p, b);
+        const float gt = p.sp.g[sd < kMaxSponge ? sd : kMaxSponge - 1];
         const double *cf = bs.coeffs + 5 * mm;
-        const bool spg = p.sp.w > 0;
-        double g = 1.0;
-        double rb = (double)rt;
-        if (spg) {
-            sponge_g32(p, b, g);
-            rb = rb / g;
-        }
+        const double c0 = cf[0], c1 = cf[1], c2 = cf[2], c3 = cf[3], c4 = cf[4];
+        // every load above is issued before any arithmetic on what they
+        // return (left to itself the compiler waits for the gather first)
+        __builtin_amdgcn_sched_barrier(0);
+        const double dn = (double)tdn, x1 = (double)tx1, x2 = (double)tx2;
+        const double y1 = (double)ty1, y2 = (double)ty2;
+        const double g = sd < p.sp.w ? (double)gt : 1.0;
+        double rb = (double)rt / g;
         // slot sp holds step n+2's adjoint taps, slot sp^1 step n+1's
         const double yb1 = sp ? ya : yc, yb2 = sp ? yc : ya;
         const double db1 = sp ? xa : xb, db2 = sp ? xb : xa;
@@ -1327,8 +1333,8 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
             for (int k = 0; k < p.nrcv; ++k)
                 if (p.rcv_bpos[k] == e) rb += p.gsig[(long long)k * p.ir_cap + n];
         const double kap = (double)bs.kconst * (double)0.4f;
-        const double yb = -0.1 * kap * rb - cf[3] * yb1 - cf[4] * yb2;
-        const double db = -0.9 * kap * rb + cf[0] * yb + cf[1] * yb1 + cf[2] * yb2;
+        const double yb = -0.1 * kap * rb - c3 * yb1 - c4 * yb2;
+        const double db = -0.9 * kap * rb + c0 * yb + c1 * yb1 + c2 * yb2;
         if (e < nb) {
             m = mm;
             gv[0] = yb * dn;
@@ -1338,39 +1344,70 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
             gv[4] = -yb * y2;
             ((double *)bs.ys[sp])[e] = yb;
             ((double *)bs.xs[sp])[e] = db;
-            R[b] = (C)(spg ? rb * g : rb);
+            R[b] = (C)(rb * g);
         }
     }
-    // per-material block sums, fixed reduction order.  Lanes without an
-    // entry hold m = -1 and zero terms; a wave whose entries share one
-    // material reduces only that material's five sums (the others are the
-    // sums of zeros either way, so the result is the same bit for bit).
+    double acc = 0.0;
+    if (tree) {
+        // Up to three materials: the 5*nmat sums of the wave in one
+        // transposed butterfly over 16 slots (slot 5*m + c holds this lane's
+        // term of coefficient c of material m).  At each of the first four
+        // steps a lane keeps half of its slots, adds its partner's copies of
+        // them and hands over the other half, so 8+4+2+1 exchanges leave lane
+        // L with the partial of slot (L >> 2) & 15; two plain steps finish it.
+        // Fixed order, so the sums are deterministic.
+        double v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = 0.0;
+#pragma unroll
+        for (int c = 0; c < 5; ++c)
+#pragma unroll
+            for (int mm = 0; mm < 3; ++mm)
+                if (m == mm) v[5 * mm + c] = gv[c];
+#pragma unroll
+        for (int h = 8, d = 32; h >= 1; h >>= 1, d >>= 1) {
+            const bool up = (lane & d) != 0;
+#pragma unroll
+            for (int q = 0; q < h; ++q) {
+                const double keep = up ? v[h + q] : v[q];
+                const double give = up ? v[q] : v[h + q];
+                v[q] = keep + __shfl_xor(give, d);
+            }
+        }
+        double t = v[0];
+        t = t + __shfl_xor(t, 2);
+        t = t + __shfl_xor(t, 1);
+        if (writer) *dst = prev + t; // lane 4s holds the sum of slot s
+        return;
+    }
+    // More materials: per-material wave sums, fixed reduction order.  Lanes
+    // without an entry hold m = -1 and zero terms; a wave whose entries share
+    // one material reduces only that material's five sums (the others are
+    // the sums of zeros either way, so the result is the same bit for bit).
+    // Every lane ends with every sum; lane 5*mm + c adds sum (mm, c) to the
+    // wave's row.  The host adds the rows in a fixed order.
     const int mw = wave_max_int(m);
     const bool uniform = __ballot(m >= 0 && m != mw) == 0ull;
     if (uniform) {
-        if (lane < 5 * bs.nmat) part[wave][lane] = 0.0;
         if (mw >= 0) {
+            double sum[5];
 #pragma unroll
-            for (int c = 0; c < 5; ++c) {
-                const double s = wave_sum(gv[c]);
-                if (lane == 0) part[wave][mw * 5 + c] = s;
-            }
+            for (int c = 0; c < 5; ++c) sum[c] = wave_sum(gv[c]);
+#pragma unroll
+            for (int c = 0; c < 5; ++c)
+                if (lane == mw * 5 + c) acc = sum[c];
         }
     } else {
         for (int mm = 0; mm < bs.nmat; ++mm) {
+            double sum[5];
 #pragma unroll
-            for (int c = 0; c < 5; ++c) {
-                const double s = wave_sum(m == mm ? gv[c] : 0.0);
-                if (lane == 0) part[wave][mm * 5 + c] = s;
-            }
+            for (int c = 0; c < 5; ++c) sum[c] = wave_sum(m == mm ? gv[c] : 0.0);
+#pragma unroll
+            for (int c = 0; c < 5; ++c)
+                if (lane == mm * 5 + c) acc = sum[c];
         }
     }
-    __syncthreads();
-    if (threadIdx.x < 5 * bs.nmat) {
-        double s = part[0][threadIdx.x];
-        for (int w = 1; w < kWaves; ++w) s = s + part[w][threadIdx.x];
-        *dst = prev + s;
-    }
+    if (writer) *dst = prev + acc;
 }
 
 // Seeds the ring of list-cell pressures from the fields before step n:

@@ -1673,7 +1673,8 @@ int virs_adjoint(virs_ctx *c, const float *dl_dir, double *grad)
         for (long long i = 0; i < (long long)c->nrcv * T; ++i) g[i] = dl_dir[i];
         HIPC(c, hipMemcpy(c->gsig, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice));
     }
-    const long long gb = std::max<long long>(1, list_blocks(c->nb)); // one row per list block
+    // one row of partial sums per wave of a list block
+    const long long gb = std::max<long long>(1, list_blocks(c->nb)) * kWaves;
     if (c->grad_blocks != gb || !c->grad_part) {
         dfree(c->grad_part);
         rc = dalloc(c, &c->grad_part, sizeof(double) * 5 * 8 * gb);
