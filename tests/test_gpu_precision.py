@@ -124,10 +124,12 @@ def test_scaled_storage_tracks_fp64_on_tiny_grid(gpu, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
-def test_fused_and_list_boundary_agree_in_scaled_storage(gpu, prec):
+@pytest.mark.parametrize("hard", [True, False])
+def test_fused_and_list_boundary_agree_in_scaled_storage(gpu, prec, hard):
     """Fusing the filter into the stencil rounds the corrected boundary value
-    once instead of twice; the two paths stay within a few storage ulps."""
-    p = tiny_problem(9, steps=80)
+    once instead of twice; the two paths stay within a few storage ulps,
+    with and without a hard-source cell."""
+    p = tiny_problem(9, steps=80, hard=hard)
     a = gpu_context(p, prec, fuse=True)
     b = gpu_context(p, prec, fuse=False)
     a.step(p["steps"])
