@@ -1082,8 +1082,9 @@ constexpr int kTailEntries = VIRS_TAIL_ENTRIES;
 // the corrected field), receiver gather, and the step counter.  Point cells
 // on the list are handled by their entry's thread, the others by block 0.
 //
-// One entry per thread (kTailEntries).  The launch is bound by latency more
-// than bytes, so every load that the entry index e names
+// One entry per thread (kTailEntries).  The launch is short and its p^{n+1}
+// accesses are scattered, so latency weighs as much as bytes: every load
+// that the entry index e names
 // (the list index, the material byte, both slots of the pressure ring hv and
 // of the filter taps) is issued at once, from e clamped into the list, and
 // none depends on the step counter: both ring slots are read every step, and
@@ -1141,8 +1142,9 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) rt[u] = R[bv[u]];
-        // before step n, slot n&1 holds step n-2's value and slot (n+1)&1
-        // step n-1's (the ring rule of BoundaryState and hv)
+        // before step n, tap slot n&1 holds step n-2's value and slot
+        // (n+1)&1 step n-1's (BoundaryState); hv[n&1] holds p^n and
+        // hv[(n+1)&1] p^{n-1} (TailParams::hv)
         const bool odd = (n & 1) != 0;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
