@@ -33,6 +33,27 @@ def test_fp64_adjoint_matches_oracle(gpu, sponge):
     f.close()
 
 
+@pytest.mark.parametrize("nmat", [3, 5])
+def test_fp64_adjoint_material_counts(gpu, nmat):
+    """Up to three materials the wave sums of the gradient go through one
+    tree over 16 slots, with more through per-material sums: both against
+    the oracle."""
+    p = tiny_problem(23, dims=(20, 16, 12), steps=60, nmat=nmat)
+    n = p["steps"]
+    print(f"fp64 adjoint, {nmat} materials", flush=True)
+    f = gpu_context(p, "fp64")
+    f.forward_record(n, 3)
+    ir = f.receivers(True)
+    target = np.random.default_rng(7).standard_normal(ir.shape) * 1e-3
+    dl = (2 * (ir - target)).astype(np.float32)
+    g = f.adjoint(dl)
+    g_or = oracle_problem(p).adjoint(n, dl.astype(np.float64))
+    assert g.shape == g_or.shape == (nmat, 5)
+    print(f"fp64 adjoint ({nmat} materials) vs oracle rel {rel_l2(g, g_or):.2e}", flush=True)
+    assert rel_l2(g, g_or) <= 1e-10
+    f.close()
+
+
 def test_fp64_adjoint_receivers_sharing_cells(gpu):
     """Receivers that share a cell, off the list (three on one cell) and on
     it (two): each receiver's dL/dIR enters the adjoint field once, as the

@@ -184,3 +184,28 @@ def test_device_rms_stop_in_one_batch(gpu, use_graph):
     np.testing.assert_array_equal(f.field(0), after["p"].astype(np.float32))
     assert f.percent_to_target_rms > 1.0
     f.close()
+
+
+@pytest.mark.parametrize("prec,nx", [("fp32", 768), ("fp32", 2048), ("fp16", 1536), ("bf16", 2560)])
+def test_chunks_ending_mid_row(gpu, prec, nx):
+    """Stencil chunks of 256 vectors that end mid-row (Nx = 768 FP32,
+    1536 16-bit) or are shorter than a row (Nx = 2048 FP32, 2560 16-bit),
+    so waves cross row ends and x-neighbours cross chunks: the oracle's
+    results bit for bit, for more than one grid size."""
+    p = tiny_problem(15, dims=(nx, 9, 10), steps=30)
+    _say(f"chunks ending mid-row {prec} nx={nx}")
+    outs = []
+    for nblk in (0, 7):
+        f = gpu_context(p, prec, dense_blocks=nblk)
+        f.step(p["steps"])
+        outs.append((f.receivers(True), f.field(0)))
+        f.close()
+    pb = oracle_problem(p)
+    if prec == "fp32":
+        ref = pb.run(p["steps"], "f32", want_fields=True)
+    else:
+        ref = pb.run16(p["steps"], prec, True, want_fields=True)
+    np.testing.assert_array_equal(outs[0][0], ref["ir"])
+    np.testing.assert_array_equal(outs[0][1], ref["p"])
+    np.testing.assert_array_equal(outs[1][0], outs[0][0])
+    np.testing.assert_array_equal(outs[1][1], outs[0][1])
