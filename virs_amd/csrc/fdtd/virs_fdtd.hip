@@ -320,7 +320,8 @@ int tail_grid(long long nb, int per_thread = 1) { return (int)(1 + list_blocks(n
 // Fewer blocks that each march more planes beat more resident waves here:
 // on the 512x384x256 room, planning for 2 / 3 / 4 / 5 / 6 blocks per CU
 // gave FP32 steps of 363 / 399 / 377 / 373 / 390 Gvox/s and FP16 steps of
-// 557 / 648 / 626 / 626 Gvox/s.  Tile mode (one block per chunk x z-range)
+// 557 / 648 / 626 / 626 Gvox/s; with the current 16-bit stencil, 2 / 3 / 4
+// blocks give FP16 613 / 710 / 666 and FP32 364 / 399 / 376.  Tile mode (one block per chunk x z-range)
 // is used when whole z-ranges of every chunk fill at least 85% of the
 // capacity in one round; otherwise the (chunk, z) planes are split into
 // `capacity` equal ranges.  A descriptor value dense_blocks > 0 forces the
