@@ -1076,6 +1076,15 @@ __device__ inline int sponge_dist32(const TailParams &p, uint32_t idx)
 #define VIRS_TAIL_ENTRIES 1
 #endif
 constexpr int kTailEntries = VIRS_TAIL_ENTRIES;
+// Threads per block of the forward list launch.  On the 512x384x256 room
+// 128 is fastest: FP16 714 against 708-710 Gvox/s per step with 256 (list
+// launch 12.0-12.4 against 12.3-12.9 us), FP32 404 against 401; 512 and
+// 1024 are slower.
+#ifndef VIRS_TAIL_BLOCK
+#define VIRS_TAIL_BLOCK 128
+#endif
+constexpr int kTailBlock = VIRS_TAIL_BLOCK;
+static_assert(kTailBlock >= 128 && kTailBlock % 64 == 0, "the point block needs 100+ threads");
 
 // The rest of a step after k_dense: the boundary biquad over the compact
 // list when it was not fused into the stencil, then soft sources (added to
@@ -1095,7 +1104,7 @@ constexpr int kTailEntries = VIRS_TAIL_ENTRIES;
 // for it where the branch rejoins.  The stores are predicated on the entry
 // being in the list and the step not skipped (stop bit).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
+__global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
 {
     using S = Store<T>;
     using C = typename S::C;
@@ -1119,7 +1128,7 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
     const long long nb = p.bs.nb;
     if (blockIdx.x > 0) { // list blocks exist only when nb > 0
         constexpr int U = kTailEntries;
-        // block b covers entries [(b-1)*U*kBlock, b*U*kBlock), lane-contiguous
+        // block b covers entries [(b-1)*U*kTailBlock, b*U*kTailBlock), lane-contiguous
         // per slot u; all loads of all U entries go out before the gathers
         long long ev[U];
         uint32_t bv[U];
@@ -1129,7 +1138,7 @@ __global__ __launch_bounds__(kBlock) void k_tail(const TailParams p)
         Y ya[U], yb[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            ev[u] = ((long long)(blockIdx.x - 1) * U + u) * kBlock + threadIdx.x;
+            ev[u] = ((long long)(blockIdx.x - 1) * U + u) * kTailBlock + threadIdx.x;
             const long long ec = ev[u] < nb ? ev[u] : nb - 1;
             bv[u] = p.bidx[ec];
             mv[u] = p.bs.bmat[ec]; // always present for the list launch

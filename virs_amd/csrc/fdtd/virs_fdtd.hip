@@ -312,8 +312,11 @@ TailParams tail_params(const virs_ctx *c, const void *P, const void *Q, void *R,
 
 // List launches: block 0 takes the points and the step counter, blocks
 // 1.. one list entry per thread.
-long long list_blocks(long long nb, int per_thread = 1) { return (nb + (long long)kBlock * per_thread - 1) / ((long long)kBlock * per_thread); }
-int tail_grid(long long nb, int per_thread = 1) { return (int)(1 + list_blocks(nb, per_thread)); }
+long long list_blocks(long long nb, int per_thread = 1, int block = kBlock)
+{
+    return (nb + (long long)block * per_thread - 1) / ((long long)block * per_thread);
+}
+int tail_grid(long long nb, int per_thread = 1, int block = kBlock) { return (int)(1 + list_blocks(nb, per_thread, block)); }
 
 // Stencil grid.  capacity = resident blocks of the whole GPU: occupancy x
 // CUs, the occupancy capped at VIRS_DENSE_OCC_CAP blocks (12 waves) per CU.
@@ -429,11 +432,11 @@ hipError_t launch_dense(const virs_ctx *c, const DenseParams &p, bool adj)
 template <typename T>
 void launch_tail_t(const virs_ctx *c, const TailParams &t)
 {
-    const int g = tail_grid(t.bs.nb, kTailEntries);
+    const int g = tail_grid(t.bs.nb, kTailEntries, kTailBlock);
     if (c->kev[0])
-        hipExtLaunchKernelGGL(k_tail<T>, dim3(g), dim3(kBlock), 0, c->stream, c->kev[0], c->kev[1], 0, t);
+        hipExtLaunchKernelGGL(k_tail<T>, dim3(g), dim3(kTailBlock), 0, c->stream, c->kev[0], c->kev[1], 0, t);
     else
-        hipLaunchKernelGGL(k_tail<T>, dim3(g), dim3(kBlock), 0, c->stream, t);
+        hipLaunchKernelGGL(k_tail<T>, dim3(g), dim3(kTailBlock), 0, c->stream, t);
 }
 
 hipError_t launch_tail(const virs_ctx *c, const TailParams &t)
