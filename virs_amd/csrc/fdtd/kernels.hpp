@@ -1085,6 +1085,15 @@ constexpr int kTailEntries = VIRS_TAIL_ENTRIES;
 #endif
 constexpr int kTailBlock = VIRS_TAIL_BLOCK;
 static_assert(kTailBlock >= 128 && kTailBlock % 64 == 0, "the point block needs 100+ threads");
+// Threads per block of the reverse list launch (one row of gradient
+// partials per wave).  128 and 256 measure alike (FP32 10.55 / 10.61 us on
+// the 256x192x128 room).
+#ifndef VIRS_TAIL_ADJ_BLOCK
+#define VIRS_TAIL_ADJ_BLOCK 256
+#endif
+constexpr int kTailAdjBlock = VIRS_TAIL_ADJ_BLOCK;
+constexpr int kTailAdjWaves = kTailAdjBlock / 64;
+static_assert(kTailAdjBlock >= 64 && kTailAdjBlock % 64 == 0, "whole waves");
 
 // The rest of a step after k_dense: the boundary biquad over the compact
 // list when it was not fused into the stencil, then soft sources (added to
@@ -1267,7 +1276,7 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
 // per-block sum through LDS put a barrier, which waits for the waves'
 // stores, at the end of every block: 17.0 against 10.0 us per launch.)
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
+__global__ __launch_bounds__(kTailAdjBlock) void k_tail_adj(const TailParams p)
 {
     using C = T;
     const long long n = p.ctl->nring[p.slot];
@@ -1294,7 +1303,7 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
         return;
     }
     // list blocks (they exist only when the list is not empty)
-    const long long e = (long long)(blockIdx.x - 1) * kBlock + threadIdx.x;
+    const long long e = (long long)(blockIdx.x - 1) * kTailAdjBlock + threadIdx.x;
     // this wave's running sums (one row per wave, so no barrier joins the
     // waves), loaded now so the load is not the last link of the chain.
     // With at most 16 sums (tree reduction below) lane 4s writes sum s,
@@ -1302,7 +1311,7 @@ __global__ __launch_bounds__(kBlock) void k_tail_adj(const TailParams p)
     const bool tree = 5 * bs.nmat <= 16;
     const int pslot = tree ? ((lane & 3) == 0 ? lane >> 2 : 64) : lane;
     const bool writer = pslot < 5 * bs.nmat;
-    double *dst = p.grad_part + ((long long)(blockIdx.x - 1) * kWaves + wave) * 5 * bs.nmat + (writer ? pslot : 0);
+    double *dst = p.grad_part + ((long long)(blockIdx.x - 1) * kTailAdjWaves + wave) * 5 * bs.nmat + (writer ? pslot : 0);
     const double prev = writer ? *dst : 0.0;
     double gv[5] = {0, 0, 0, 0, 0};
     int m = -1;

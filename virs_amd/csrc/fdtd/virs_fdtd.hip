@@ -481,11 +481,11 @@ int stash_boundary(virs_ctx *c, long long n)
 
 hipError_t launch_tail_adj(const virs_ctx *c, const TailParams &t)
 {
-    const int g = tail_grid(t.bs.nb);
+    const int g = tail_grid(t.bs.nb, 1, kTailAdjBlock);
     if (c->d.precision == VIRS_FP64)
-        hipLaunchKernelGGL(k_tail_adj<double>, dim3(g), dim3(kBlock), 0, c->stream, t);
+        hipLaunchKernelGGL(k_tail_adj<double>, dim3(g), dim3(kTailAdjBlock), 0, c->stream, t);
     else
-        hipLaunchKernelGGL(k_tail_adj<float>, dim3(g), dim3(kBlock), 0, c->stream, t);
+        hipLaunchKernelGGL(k_tail_adj<float>, dim3(g), dim3(kTailAdjBlock), 0, c->stream, t);
     return hipGetLastError();
 }
 
@@ -1678,7 +1678,7 @@ int virs_adjoint(virs_ctx *c, const float *dl_dir, double *grad)
         HIPC(c, hipMemcpy(c->gsig, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice));
     }
     // one row of partial sums per wave of a list block
-    const long long gb = std::max<long long>(1, list_blocks(c->nb)) * kWaves;
+    const long long gb = std::max<long long>(1, list_blocks(c->nb, 1, kTailAdjBlock)) * kTailAdjWaves;
     if (c->grad_blocks != gb || !c->grad_part) {
         dfree(c->grad_part);
         rc = dalloc(c, &c->grad_part, sizeof(double) * 5 * 8 * gb);
