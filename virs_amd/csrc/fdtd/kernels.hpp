@@ -782,13 +782,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                     sum = sum + PM[j];
                     const C kw = (C)(((j & 1) ? kwo : kwe) >> (8 * (j >> 1)) & 0xffu); // cells that update
                     // t = sum - kw*pc, u = a*pc - b*q, R = t*(g*c2dt2*ih2) + u.
-                    // With stochastic rounding u and R are fused multiply-adds
-                    // (each rounds once).  Nearest rounding keeps every product
-                    // rounded apart: its rounding errors do not average out, and
-                    // the fused final step drove the room's constant mode
-                    // (relative IR error 0.36 against 5e-3 on the 2000-step
-                    // shoebox).
-                    const C t = sum - kw * pcv;
+                    // pc is a 16-bit value (at most 11 significant bits) and kw
+                    // an integer up to 6, so kw*pc is exact in FP32 and t as
+                    // one fused multiply-add has the bits of the product and
+                    // difference rounded apart; likewise 2*pc - q.  With
+                    // stochastic rounding R is a fused multiply-add as well.
+                    // Nearest rounding keeps R's product rounded apart: its
+                    // rounding errors do not average out, and the fused final
+                    // step drove the room's constant mode (relative IR error
+                    // 0.36 against 5e-3 on the 2000-step shoebox).
+                    const C t = __builtin_fmaf(-kw, pcv, sum);
                     C a = (C)2, b = (C)1, g = (C)1;
                     if constexpr (SPONGE) {
                         const int d = edge_dist(x0 + j, y, z, nx, ny, nz);
@@ -808,7 +811,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                         else
                             upd = __builtin_fmaf(t, kf, __builtin_fmaf((C)2, pcv, -qv[j]));
                     } else {
-                        upd = (a * pcv - b * qv[j]) + g * (t * kf);
+                        if constexpr (SPONGE)
+                            upd = (a * pcv - b * qv[j]) + g * (t * kf);
+                        else
+                            upd = __builtin_fmaf((C)2, pcv, -qv[j]) + t * kf;
                     }
                     o[j] = HARD ? bsel(bit_mask(c, 3), bsel(bit_mask(c, 0), hard_u, (C)0), upd) : upd;
                     continue;
