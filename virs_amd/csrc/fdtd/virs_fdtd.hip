@@ -1047,7 +1047,6 @@ int virs_step(virs_ctx *c, int64_t nsteps, int32_t *keep_going)
     if (nsteps < 0) return VIRS_ERR_ARG;
     if (keep_going) *keep_going = c->stopped ? 0 : 1;
     if (c->stopped) return VIRS_OK;
-    const long long first = c->steps;
     long long left = nsteps;
     const int gsteps = 30; // a multiple of 3 keeps the buffer rotation phase
     if (c->d.use_graph) {
