@@ -294,6 +294,28 @@ def test_fp16_nearest_matches_numpy():
     np.testing.assert_array_equal(back, want.astype(np.float32))
 
 
+def test_16bit_products_in_the_update_are_exact_in_fp32():
+    """kw*pc (kw = 0..6) and 2*pc are exact in FP32 for every finite FP16
+    value and every BF16 value below 2^100 (the per-field scale keeps stored
+    values near 2^14).  The 16-bit stencil relies on this when it computes
+    sum - kw*pc and 2*pc - q as fused multiply-adds: each then has the bits
+    of the product and the difference rounded apart, as the oracle's runner
+    computes them."""
+    h = np.arange(65536, dtype=np.uint16).view(np.float16).astype(np.float64)
+    h = h[np.isfinite(h)]
+    b32 = (np.arange(65536, dtype=np.uint32) << 16).astype(np.uint32).view(np.float32)
+    b = b32[np.isfinite(b32)].astype(np.float64)
+    b = b[np.abs(b) < 2.0 ** 100]
+    for v in (h, b):
+        for kw in range(7):
+            prod = kw * v
+            np.testing.assert_array_equal(prod.astype(np.float32).astype(np.float64), prod)
+        np.testing.assert_array_equal((2 * v).astype(np.float32).astype(np.float64), 2 * v)
+    # a general FP32 value is not enough: the claim needs the short significand
+    f = np.float64(np.float32(1.0) + np.float32(2.0 ** -23))
+    assert np.float64(np.float32(3 * f)) != 3 * f
+
+
 def test_fp16_every_pattern_round_trips():
     pat = np.arange(0x10000, dtype=np.uint32).astype(np.uint16)
     h = pat.view(np.float16)
