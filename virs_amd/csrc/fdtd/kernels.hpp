@@ -1085,12 +1085,14 @@ constexpr int kTailEntries = VIRS_TAIL_ENTRIES;
 // Threads per block of the forward list launch.  On the 512x384x256 room
 // 128 is fastest: FP16 714 against 708-710 Gvox/s per step with 256 (list
 // launch 12.0-12.4 against 12.3-12.9 us), FP32 404 against 401; 512 and
-// 1024 are slower.
+// 1024 are slower, 64 the same or slightly slower (FP16 714-715 against
+// 717-718).  Block 0's point work strides over its threads, so any whole
+// number of waves works.
 #ifndef VIRS_TAIL_BLOCK
 #define VIRS_TAIL_BLOCK 128
 #endif
 constexpr int kTailBlock = VIRS_TAIL_BLOCK;
-static_assert(kTailBlock >= 128 && kTailBlock % 64 == 0, "the point block needs 100+ threads");
+static_assert(kTailBlock >= 64 && kTailBlock % 64 == 0, "whole waves");
 // Threads per block of the reverse list launch (one row of gradient
 // partials per wave).  128 and 256 measure alike (FP32 10.55 / 10.61 us on
 // the 256x192x128 room).
@@ -1218,11 +1220,11 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
     StepCtl *ctl = p.ctl;
     const bool rms = ctl->es_enable && p.nrcv > 0 && n > kRmsMinStep && n <= p.ir_cap;
     __shared__ float win[kRmsWindow];
-    if (rms && k < kRmsWindow) win[k] = (float)p.ir[n - kRmsWindow + k];
-    if (k < p.nsoft && in_sig) {
-        const long long i = p.soft_idx[k], bp = p.soft_bpos[k];
-        const int j0 = p.src_rcv[k];
-        const C sg = (C)p.soft_sig[(long long)k * p.soft_len + n];
+    for (int q = k; rms && q < kRmsWindow; q += kTailBlock) win[q] = (float)p.ir[n - kRmsWindow + q];
+    for (int ks = k; ks < p.nsoft && in_sig; ks += kTailBlock) {
+        const long long i = p.soft_idx[ks], bp = p.soft_bpos[ks];
+        const int j0 = p.src_rcv[ks];
+        const C sg = (C)p.soft_sig[(long long)ks * p.soft_len + n];
         C r = S::ld(R[i]);
         if constexpr (S::kScaled) r = r * sRi;
         r = r + sg;
@@ -1236,12 +1238,12 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
             }
         }
     }
-    if (k < p.nrcv && in_ir) {
-        const long long i = p.rcv_idx[k], bp = p.rcv_bpos[k];
-        const bool by_source = in_sig && p.rcv_src[k] >= 0;
+    for (int kr = k; kr < p.nrcv && in_ir; kr += kTailBlock) {
+        const long long i = p.rcv_idx[kr], bp = p.rcv_bpos[kr];
+        const bool by_source = in_sig && p.rcv_src[kr] >= 0;
         C r = S::ld(R[i]);
         if constexpr (S::kScaled) r = r * sRi;
-        if (bp < 0 && !by_source) p.ir[(long long)k * p.ir_cap + n] = (double)r;
+        if (bp < 0 && !by_source) p.ir[(long long)kr * p.ir_cap + n] = (double)r;
     }
     // RMS rule of step n over receiver 0's samples [n-100, n), in float in
     // the reference's order (simulator_gpu.cu:303-330)
