@@ -1040,6 +1040,31 @@ static int build_graph(virs_ctx *c, int steps)
     return VIRS_OK;
 }
 
+// n plain steps (receivers on, no trace) on the context's stream: replays of
+// the 30-step graph of the current rotation phase when the context uses
+// graphs, single launches for the rest.
+static int run_steps(virs_ctx *c, long long n)
+{
+    const int gsteps = 30; // a multiple of 3 keeps the buffer rotation phase
+    if (c->d.use_graph) {
+        while (n >= gsteps) {
+            if (!c->graph[c->ip] || c->graph_steps != gsteps) {
+                const int rc = build_graph(c, gsteps);
+                if (rc) return rc;
+            }
+            HIPC(c, hipGraphLaunch(c->graph[c->ip], c->stream));
+            n -= gsteps;
+            c->steps += gsteps;
+        }
+    }
+    for (; n > 0; --n) {
+        const int rc = launch_step(c);
+        if (rc) return rc;
+        ++c->steps;
+    }
+    return VIRS_OK;
+}
+
 int virs_step(virs_ctx *c, int64_t nsteps, int32_t *keep_going)
 {
     int rc = check_ready(c);
@@ -1047,24 +1072,8 @@ int virs_step(virs_ctx *c, int64_t nsteps, int32_t *keep_going)
     if (nsteps < 0) return VIRS_ERR_ARG;
     if (keep_going) *keep_going = c->stopped ? 0 : 1;
     if (c->stopped) return VIRS_OK;
-    long long left = nsteps;
-    const int gsteps = 30; // a multiple of 3 keeps the buffer rotation phase
-    if (c->d.use_graph) {
-        while (left >= gsteps) {
-            if (!c->graph[c->ip] || c->graph_steps != gsteps) {
-                rc = build_graph(c, gsteps);
-                if (rc) return rc;
-            }
-            HIPC(c, hipGraphLaunch(c->graph[c->ip], c->stream));
-            left -= gsteps;
-            c->steps += gsteps;
-        }
-    }
-    for (; left > 0; --left) {
-        rc = launch_step(c);
-        if (rc) return rc;
-        ++c->steps;
-    }
+    rc = run_steps(c, nsteps);
+    if (rc) return rc;
     c->counted = c->steps;
     if (c->es_enable && c->nrcv > 0 && c->steps > (long long)kRmsMinStep + 1) {
         // The device evaluates the RMS rule at the end of every step and
@@ -1600,14 +1609,18 @@ int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
     if (rc) return rc;
     const long long last_base = (long long)(ncp - 1) * c->seg;
     const int last_buf = (ncp - 1) & 1;
-    for (long long s = 0; s < nsteps; ++s) {
-        if (s % c->seg == 0) {
-            rc = save_checkpoint(c, (int)(s / c->seg));
-            if (!rc && s == last_base) rc = seed_trace(c, last_buf, s);
-            if (rc) return rc;
-        }
-        if (s >= last_base) rc = launch_step(c, c->trace_d[last_buf], c->trace_y[last_buf], last_base, true);
-        else rc = launch_step(c);
+    // segments before the last: a checkpoint, then plain steps (graphed like
+    // virs_step's); the last segment records its boundary trace step by step
+    for (int k = 0; k + 1 < ncp; ++k) {
+        rc = save_checkpoint(c, k);
+        if (!rc) rc = run_steps(c, c->seg);
+        if (rc) return rc;
+    }
+    rc = save_checkpoint(c, ncp - 1);
+    if (!rc) rc = seed_trace(c, last_buf, last_base);
+    if (rc) return rc;
+    for (long long s = last_base; s < nsteps; ++s) {
+        rc = launch_step(c, c->trace_d[last_buf], c->trace_y[last_buf], last_base, true);
         if (rc) return rc;
         ++c->steps;
     }
