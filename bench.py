@@ -480,7 +480,35 @@ def main():
         out["layer_images"] = layer_images(prep)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(prep)
+    out["targets"] = targets(out, args.precision)
     print(json.dumps(out), flush=True)
+
+
+def targets(out, main_precision):
+    """The four goals of the FP32/FP16/adjoint measurements in this run, each
+    with its measured value, its bar and whether it is met (legs that did not
+    run are left out)."""
+    t = {}
+    if main_precision == "fp32":
+        frac = out["roofline"]["frac"]
+        t["fp32_stencil_hbm_frac"] = {"value": frac, "bar": ">= 0.70", "met": frac >= 0.70}
+        v = out.get("variants", {}).get("fp16")
+        if v:
+            t["fp16_speedup_vs_fp32"] = {"value": v["speedup_vs_main"], "bar": ">= 1.7",
+                                         "met": v["speedup_vs_main"] >= 1.7}
+    pe = out.get("precision_error")
+    if pe:
+        e = pe["fp16"]["rel_l2_max"]
+        t["fp16_ir_rel_l2_vs_fp64"] = {"value": e, "bar": "<= 1e-3", "met": e <= 1e-3}
+    adj = out.get("adjoint")
+    if adj:
+        c = adj["fp32"]["cost_vs_forward"]
+        t["adjoint_cost_vs_forward"] = {"value": c, "bar": "<= 3", "met": c <= 3.0,
+                                        "whole_trace": adj.get("fp32_whole_trace", {}).get("cost_vs_forward")}
+        err = adj.get("rel_err_fp32_vs_fp64")
+        if err is not None:
+            t["adjoint_fp32_vs_fp64_rel_err"] = {"value": err, "bar": "<= 1e-4", "met": err <= 1e-4}
+    return t
 
 
 if __name__ == "__main__":
