@@ -65,6 +65,9 @@ typedef struct {
     int32_t nrcv;
     const int64_t *rcv_idx;         /* [nrcv] */
     const double *init_p, *init_q;  /* optional initial p^0 and p^-1, NULL = zero */
+    double *state_io;               /* optional [nb][4] filter state x1 x2 y1 y2: read at the
+                                     * start of an FP32/FP64 run, written at its end, so a run
+                                     * can continue in chunks (with init_p/init_q and n0) */
 } oracle_desc;
 
 static int g_threads = 1;
@@ -119,6 +122,11 @@ int oracle_run_##SUFFIX(const oracle_desc *d, int64_t n0, int64_t nsteps, double
     }                                                                                     \
     if (d->init_p) for (int64_t i = 0; i < N; ++i) P[i] = (T)d->init_p[i];                \
     if (d->init_q) for (int64_t i = 0; i < N; ++i) Q[i] = (T)d->init_q[i];                \
+    if (d->state_io)                                                                      \
+        for (int64_t k = 0; k < d->nb * 4; ++k) {                                         \
+            st[k] = d->state_io[k];                                                       \
+            stf[k] = (float)d->state_io[k];                                               \
+        }                                                                                 \
     const T c2dt2 = (T)d->c2dt2, ih2 = (T)d->ih2;                                         \
     for (int64_t s = 0; s < nsteps; ++s) {                                                \
         const int64_t n = n0 + s;                                                         \
@@ -192,6 +200,9 @@ int oracle_run_##SUFFIX(const oracle_desc *d, int64_t n0, int64_t nsteps, double
     }                                                                                     \
     if (final_p) for (int64_t i = 0; i < N; ++i) final_p[i] = (double)P[i];               \
     if (final_q) for (int64_t i = 0; i < N; ++i) final_q[i] = (double)Q[i];               \
+    if (d->state_io)                                                                      \
+        for (int64_t k = 0; k < d->nb * 4; ++k)                                           \
+            d->state_io[k] = d->state_f32 ? (double)stf[k] : st[k];                       \
     free(P); free(Q); free(R); free(st); free(stf);                                       \
     return 0;                                                                             \
 }

@@ -35,6 +35,7 @@ class OracleDesc(C.Structure):
         ("soft_len", C.c_int64),
         ("nrcv", C.c_int32), ("rcv_idx", C.c_void_p),
         ("init_p", C.c_void_p), ("init_q", C.c_void_p),
+        ("state_io", C.c_void_p),
     ]
 
 
@@ -130,6 +131,8 @@ class Problem:
             np.zeros(1, f32), np.zeros(1, f32), np.zeros(1, f32))
         self.init_p = None if init_p is None else np.ascontiguousarray(init_p, np.float64)
         self.init_q = None if init_q is None else np.ascontiguousarray(init_q, np.float64)
+        # optional [nb*4] filter state carried between chunked FP32/FP64 runs
+        self.state_io = None
 
     def desc(self):
         d = OracleDesc()
@@ -154,6 +157,7 @@ class Problem:
         d.rcv_idx = self.rcv_idx.ctypes.data
         d.init_p = None if self.init_p is None else self.init_p.ctypes.data
         d.init_q = None if self.init_q is None else self.init_q.ctypes.data
+        d.state_io = None if self.state_io is None else self.state_io.ctypes.data
         self._keep = d
         return d
 

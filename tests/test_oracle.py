@@ -391,3 +391,24 @@ def test_16bit_runner_tracks_the_fp64_scheme(fmt, bound):
     ref = oracle_problem(p).run(p["steps"], "f64")["ir"]
     got = oracle_problem(p).run16(p["steps"], fmt, True)["ir"]
     assert rel_l2(got, ref) <= bound
+
+
+@pytest.mark.parametrize("precision,state_f32", [("f32", False), ("f64", False), ("f32", True)])
+def test_chunked_runs_equal_one_run(precision, state_f32):
+    """A run continued in chunks, carrying the fields (init_p/init_q), the
+    filter state (state_io) and the step offset (n0), gives the bits of one
+    uninterrupted run."""
+    p = tiny_problem(41, steps=60)
+    whole = oracle_problem(p, state_f32=state_f32).run(60, precision, want_fields=True)
+    pb = oracle_problem(p, state_f32=state_f32)
+    pb.state_io = np.zeros(4 * max(1, len(p["bidx"])), np.float64)
+    irs = []
+    for n0 in (0, 25, 40):
+        n = {0: 25, 25: 15, 40: 20}[n0]
+        out = pb.run(n, precision, n0=n0, want_fields=True)
+        irs.append(out["ir"])
+        pb.init_p = np.ascontiguousarray(out["p"].reshape(-1), np.float64)
+        pb.init_q = np.ascontiguousarray(out["q"].reshape(-1), np.float64)
+    np.testing.assert_array_equal(np.concatenate(irs, axis=1), whole["ir"])
+    np.testing.assert_array_equal(out["p"], whole["p"])
+    np.testing.assert_array_equal(out["q"], whole["q"])
