@@ -492,6 +492,10 @@ def targets(out, main_precision):
     if main_precision == "fp32":
         frac = out["roofline"]["frac"]
         t["fp32_stencil_hbm_frac"] = {"value": frac, "bar": ">= 0.70", "met": frac >= 0.70}
+        sf = out.get("step_hbm_frac")
+        if sf is not None:
+            # the whole step (stencil + list launch + points), algorithmic bytes over step time
+            t["fp32_step_hbm_frac"] = {"value": sf, "bar": ">= 0.70", "met": sf >= 0.70}
         v = out.get("variants", {}).get("fp16")
         if v:
             t["fp16_speedup_vs_fp32"] = {"value": v["speedup_vs_main"], "bar": ">= 1.7",
