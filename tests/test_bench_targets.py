@@ -1,0 +1,36 @@
+"""bench.py's targets summary, from a made-up bench line (no GPU)."""
+import importlib.util
+import os
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _bench():
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_targets_report_each_goal_and_skip_missing_legs():
+    b = _bench()
+    line = {
+        "roofline": {"frac": 0.75},
+        "step_hbm_frac": 0.69,
+        "variants": {"fp16": {"speedup_vs_main": 1.8}},
+        "precision_error": {"fp16": {"rel_l2_max": 4.5e-3}},
+        "adjoint": {"fp32": {"cost_vs_forward": 2.9}, "fp32_whole_trace": {"cost_vs_forward": 2.2},
+                    "rel_err_fp32_vs_fp64": 1.6e-6},
+    }
+    t = b.targets(line, "fp32")
+    assert t["fp32_stencil_hbm_frac"]["met"] is True
+    assert t["fp32_step_hbm_frac"]["met"] is False
+    assert t["fp16_speedup_vs_fp32"]["met"] is True
+    assert t["fp16_ir_rel_l2_vs_fp64"]["met"] is False
+    assert t["adjoint_cost_vs_forward"] == {"value": 2.9, "bar": "<= 3", "met": True, "whole_trace": 2.2}
+    assert t["adjoint_fp32_vs_fp64_rel_err"]["met"] is True
+    # legs that did not run, or a main precision other than FP32, leave goals out
+    t2 = b.targets({"roofline": {"frac": 0.6}}, "fp16")
+    assert t2 == {}
+    t3 = b.targets({"roofline": {"frac": 0.6}}, "fp32")
+    assert set(t3) == {"fp32_stencil_hbm_frac"} and t3["fp32_stencil_hbm_frac"]["met"] is False
