@@ -49,6 +49,9 @@ extern "C" {
 
 #define VIRS_MAX_SPONGE 64
 #define VIRS_MAX_POINTS 64
+/* Cells per grid: boundary-list entries are uint32 cell indices, as in the
+ * reference's list (simulator.cpp:571-605, simulator_dispatch.cpp:120-130). */
+#define VIRS_MAX_CELLS 4294967296ULL
 
 typedef struct virs_ctx virs_ctx;
 
@@ -76,7 +79,8 @@ typedef struct {
  * boundary list as its own launch, stochastic rounding of 16-bit fields. */
 int virs_desc_default(virs_desc *d);
 
-/* fdtd_gpu_setup (simulator_gpu.cpp:7-235): allocate the device grid, zeroed. */
+/* fdtd_gpu_setup (simulator_gpu.cpp:7-235): allocate the device grid, zeroed.
+ * VIRS_ERR_ARG for a dimension below 3 or a grid above VIRS_MAX_CELLS cells. */
 int virs_create(const virs_desc *d, virs_ctx **out);
 /* fdtd_gpu_cleanup (simulator_gpu.cpp:237-305). */
 int virs_destroy(virs_ctx *ctx);

@@ -32,6 +32,22 @@ def test_headers_declare_exactly_the_exported_entry_points():
         assert declared == set(syms), (hdr, declared ^ set(syms))
 
 
+@pytest.mark.parametrize("dims", [(2, 64, 64), (64, 64, 2), (1 << 31, 3, 3), (65536, 65536, 2),
+                                  (65536, 65536, 3), (1 << 30, 1 << 30, 1 << 30), (-5, 64, 64)])
+def test_create_rejects_grids_outside_the_limits(dims):
+    """Dimensions below 3 and grids above 2^32 cells (the boundary list's
+    uint32 index range) fail with VIRS_ERR_ARG before any device call, so
+    this runs without a GPU.  The 2^90-cell case would overflow int64."""
+    import ctypes as C
+    L = _lib.hip()
+    d = _lib.VirsDesc()
+    assert L.virs_desc_default(C.byref(d)) == 0
+    d.nx, d.ny, d.nz = dims
+    ctx = C.c_void_p(1)
+    assert L.virs_create(C.byref(d), C.byref(ctx)) == -1
+    assert not ctx.value
+
+
 @pytest.mark.parametrize("seed", range(6))
 def test_voxelize_matches_numpy_restatement(seed):
     rng = np.random.default_rng(seed)

@@ -642,7 +642,8 @@ int virs_create(const virs_desc *d, virs_ctx **out)
     c->nz = d->nz;
     if (c->nx < 3 || c->ny < 3 || c->nz < 3 || c->nx > (1 << 30) || c->ny > (1 << 30) ||
         c->nz > (1 << 30) || !(d->h > 0.f) || !(d->c > 0.f) || d->precision < 0 ||
-        d->precision > 3 || d->sponge_width < 0 || d->sponge_width > VIRS_MAX_SPONGE) {
+        d->precision > 3 || d->sponge_width < 0 || d->sponge_width > VIRS_MAX_SPONGE ||
+        (unsigned long long)c->nx * (unsigned long long)c->ny > VIRS_MAX_CELLS / (unsigned long long)c->nz) {
         delete c;
         return VIRS_ERR_ARG;
     }
