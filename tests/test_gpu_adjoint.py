@@ -80,6 +80,42 @@ def test_fp64_adjoint_receivers_sharing_cells(gpu):
     f.close()
 
 
+def test_adjoint_after_the_boundary_list_grows(gpu):
+    """Record and adjoint on every other list entry, then the whole list
+    uploaded into the same context: the checkpointed filter states, traces
+    and reverse filter states follow the new length, the stale recording is
+    refused, and the new gradient equals a fresh context's bit for bit.  A
+    ninth material after recording is refused too."""
+    from virs_amd.fdtd import FDTDError
+    p = tiny_problem(26, dims=(20, 16, 12), steps=60, sponge=2)
+    n = p["steps"]
+    half = dict(p, bidx=p["bidx"][::2], bmat=p["bmat"][::2])
+    dl = np.random.default_rng(8).standard_normal((len(p["rcv_idx"]), n)).astype(np.float32)
+    print(f"fp64 adjoint, list of {len(half['bidx'])} then {len(p['bidx'])} entries", flush=True)
+    f = gpu_context(half, "fp64")
+    f.forward_record(n, 4)
+    g_half = f.adjoint(dl)
+    f.upload_boundary(p["bidx"], p["bmat"])
+    with pytest.raises(FDTDError, match="forward_record first"):
+        f.adjoint(dl)
+    f.start(n)
+    f.forward_record(n, 4)
+    g = f.adjoint(dl)
+    fresh = gpu_context(p, "fp64")
+    fresh.forward_record(n, 4)
+    np.testing.assert_array_equal(g, fresh.adjoint(dl))
+    g_or = oracle_problem(p).adjoint(n, dl.astype(np.float64))
+    print(f"after the re-upload vs oracle rel {rel_l2(g, g_or):.2e}", flush=True)
+    assert rel_l2(g, g_or) <= 1e-10
+    assert not np.array_equal(g, g_half)
+    f.forward_record(n, 4)
+    f.set_materials(np.tile(p["coeffs"][:1], (9, 1)))
+    with pytest.raises(FDTDError, match="up to 8 materials"):
+        f.adjoint(dl)
+    f.close()
+    fresh.close()
+
+
 def test_fp32_adjoint_within_1e4_of_fp64(gpu):
     p = tiny_problem(22, dims=(22, 16, 14), steps=100, sponge=2)
     n = p["steps"]

@@ -874,6 +874,18 @@ int virs_upload_boundary(virs_ctx *c, const uint32_t *idx, const uint8_t *mat, i
     dfree(c->bmat);
     for (auto &p : c->st) dfree(p);
     for (auto &p : c->hv) dfree(p);
+    // the adjoint's list-sized buffers (checkpointed filter states, traces,
+    // reverse filter states) follow the new list length at the next record
+    // and adjoint; a recorded run no longer matches the list
+    for (auto &p : c->cp_state) dfree(p);
+    c->cp_state.clear();
+    for (int k = 0; k < 2; ++k) {
+        dfree(c->trace_d[k]);
+        dfree(c->trace_y[k]);
+    }
+    c->trace_slots = 0;
+    for (auto &p : c->ast) dfree(p);
+    c->rec_steps = 0;
     c->nb = nb;
     c->h_bidx.assign(idx, idx + nb);
     if (mat) c->h_bmat.assign(mat, mat + nb);
@@ -1670,6 +1682,7 @@ int virs_adjoint(virs_ctx *c, const float *dl_dir, double *grad)
     if (rc) return rc;
     if (!grad || (c->nrcv && !dl_dir)) return VIRS_ERR_ARG;
     if (c->rec_steps <= 0 || c->ncp <= 0) return fail(c, VIRS_ERR_STATE, "virs_forward_record first");
+    if (c->nmat > 8) return fail(c, VIRS_ERR_UNSUPPORTED, "adjoint supports up to 8 materials");
     const long long T = c->rec_steps;
     // adjoint buffers, second stream and its events
     for (int k = 0; k < 3 && !rc; ++k)
