@@ -6,18 +6,23 @@ One step = one leapfrog update of every cell of the 512x384x256 grid
 (50.3 M cells) plus the boundary-biquad list, the Gaussian-derivative source
 and 8 receivers, with fields already resident in HBM.  Prints one JSON line.
 
+Timed region: exactly the K steps as virs_step runs them (replays of a
+30-step HIP graph and one graph of the remaining steps; --no-graph: single
+launches), between two HIP events on the library's stream; config.launch
+says what the library reports it launched.
 roofline: the stencil launch's algorithmic bytes N*(3s+c) (read p^n and
 p^{n-1}, write p^{n+1}, read the cell code: c = 1 byte per cell, half a
-byte for 16-bit storage) divided by its mean launch time, taken with HIP
-events that the two launches of every 31st step of the timed region record
-at their own start and stop (hipExtLaunchKernel; the steps between are
-replayed as 30-step HIP graphs; --no-graph launches every step and samples
-about 64), on the library's own stream.
-step_hbm_frac: the whole step's algorithmic bytes (stencil, list launch,
-receivers) per step time; step_hbm_frac_fp64_state_formula prices the same
-step with N(3s+1) + Nb(69+4s) + R s instead.
-cpu_baseline: the CPU oracle (the same scheme, OpenMP over z) on a 128^3
-corner crop of the same room, on this host's cores.
+byte for 16-bit storage) divided by its mean launch time, taken over an
+untimed sampling pass of 32 single steps before the timed region, whose
+launches record their own start and stop (hipExtLaunchKernel).
+step_hbm_frac: the whole step's algorithmic bytes (stencil, boundary
+filter, receivers) per step time; step_hbm_frac_fp64_state_formula prices
+the same step with N(3s+1) + Nb(69+4s) + R s instead.
+cpu_baseline: the reference's own CPU step (simulator_cpu.cpp:55-81,
+restated in C with OpenMP over z) on a 128^3 corner crop of the same room,
+on this host's cores; the CPU oracle of the GPU scheme is reported beside it.
+error_vs_fp64: the 512x384x256 room run for its full 20000 steps in FP32,
+FP16 and BF16 storage against its FP64 run (8 receivers).
 """
 import argparse
 import json
@@ -35,6 +40,8 @@ METRIC = "Gvoxel-updates/s + % HBM peak; FP16-store vs FP32 at ≤1e-3 IR err vs
 
 
 PROFILE = os.path.join(ROOT, "profiles", "k_dense_random_512x384x256.md")
+# single steps of the untimed sampling pass that gives the launch durations
+SAMPLE_STEPS = 32
 
 
 def measured_traffic(precision):
@@ -99,12 +106,12 @@ def run_gpu(prep, precision, steps, warmup, dense_blocks=0, fuse=False, vw=0, gr
               vector_width=vw, rounding=rounding)
     f.synchronize()
     setup_s = time.perf_counter() - ts
-    f.start(warmup + steps)
+    f.start(warmup + SAMPLE_STEPS + steps)
     if warmup:
         f.step(warmup)
     f.synchronize()
     t0 = time.perf_counter()
-    total_ms, dense_ms, bnd_ms = f.time_steps(steps)
+    tm = f.time_steps(steps, SAMPLE_STEPS)
     f.synchronize()
     wall = time.perf_counter() - t0
     tr = time.perf_counter()
@@ -112,8 +119,21 @@ def run_gpu(prep, precision, steps, warmup, dense_blocks=0, fuse=False, vw=0, gr
     read_s = time.perf_counter() - tr
     mem = f.memory_bytes()
     f.close()
-    return dict(total_ms=total_ms, dense_ms=dense_ms, bnd_ms=bnd_ms, wall_s=wall, ir=ir,
-                mem=mem, s=BYTES[precision], setup_s=setup_s, read_s=read_s)
+    return dict(total_ms=tm["total_ms"], dense_ms=tm["dense_ms"], bnd_ms=tm["boundary_ms"], timing=tm,
+                wall_s=wall, ir=ir, mem=mem, s=BYTES[precision], setup_s=setup_s, read_s=read_s)
+
+
+def launch_label(tm):
+    """What the library reports the timed region launched."""
+    parts = []
+    if tm["graph_steps"]:
+        n = tm["graph_launches"] - (1 if tm["tail_graph_steps"] else 0)
+        parts.append(f"{n} replays of a {tm['graph_steps']}-step HIP graph")
+    if tm["tail_graph_steps"]:
+        parts.append(f"one {tm['tail_graph_steps']}-step HIP graph")
+    if tm["single_steps"]:
+        parts.append(f"{tm['single_steps']} single-launch steps")
+    return " + ".join(parts)
 
 
 def cpu_baseline(prep, crop=128, steps=200, warmup=5):
@@ -125,22 +145,26 @@ def cpu_baseline(prep, crop=128, steps=200, warmup=5):
     sc = prep.scenario
     n = crop ** 3
     centre = (crop // 2 * crop + crop // 2) * crop + crop // 2
+    # the reference's own CPU step (simulator_cpu.cpp:55-81, a damped scheme
+    # without boundary filters) on the crop: warm-up, then the timed steps
+    O.refcpu_steps(flags, sc.h, warmup)
+    t1 = time.perf_counter()
+    O.refcpu_steps(flags, sc.h, steps)
+    tr = time.perf_counter() - t1
+    # the GPU scheme restated on the CPU (the oracle), same crop
     pb = O.Problem(flags, sc.h, dt=prep.dt, bidx=bidx, bmat=bmat, coeffs=sc.coeffs,
                    soft_idx=[centre], soft_sig=prep.src_sig[:1, : warmup + steps], rcv_idx=[centre])
     pb.run(warmup, "f32")
     t0 = time.perf_counter()
     pb.run(steps, "f32")
     t = time.perf_counter() - t0
-    # the reference's own CPU step (a different, damped scheme) on the same crop
-    t1 = time.perf_counter()
-    O.refcpu_steps(flags, sc.h, steps)
-    tr = time.perf_counter() - t1
     return {
-        "value": round(n * steps / t / 1e9, 4), "unit": "Gvoxel-updates/s", "cores": cores,
+        "value": round(n * steps / tr / 1e9, 4), "unit": "Gvoxel-updates/s", "cores": cores,
         "kind": "port",
         "sample": f"{crop}^3 corner crop of the 512x384x256 room, {steps} steps after {warmup} "
-                  f"warm-up, FP32 oracle (same scheme, OpenMP over z)",
-        "reference_cpu_step_gvox_s": round(n * steps / tr / 1e9, 4),
+                  f"warm-up: the reference's CPU step (simulator_cpu.cpp:55-81) restated in C, "
+                  f"OpenMP over z",
+        "oracle_same_scheme_gvox_s": round(n * steps / t / 1e9, 4),
     }
 
 
@@ -174,6 +198,37 @@ def precision_error():
             keep = b > -60.0
             dev = max(dev, float(np.max(np.abs(a[keep] - b[keep]))))
         out[k] = {"rel_l2_max": float(f"{rel:.3e}"), "edc_dev_db_max": round(dev, 4)}
+    return out
+
+
+def ir_errors(irs, ref):
+    """Largest and median relative L2 over the receivers, and the largest
+    energy-decay-curve deviation (dB, where the FP64 curve is above -60 dB)."""
+    rel = [float(np.linalg.norm(irs[r] - ref[r]) / np.linalg.norm(ref[r])) for r in range(len(ref))]
+    dev = 0.0
+    for r in range(len(ref)):
+        a, b = _edc_db(irs[r]), _edc_db(ref[r])
+        keep = b > -60.0
+        dev = max(dev, float(np.max(np.abs(a[keep] - b[keep]))))
+    return {"rel_l2_max": float(f"{max(rel):.3e}"), "rel_l2_median": float(f"{np.median(rel):.3e}"),
+            "edc_dev_db_max": round(dev, 4)}
+
+
+def room_error_vs_fp64(prep, steps):
+    """The benchmark room itself: FP32, FP16 and BF16 storage against its FP64
+    run over `steps` steps at all receivers (graphed, as in the timed loop)."""
+    from virs_amd.fdtd import setup
+    irs = {}
+    for prec in ("fp64", "fp32", "fp16", "bf16"):
+        f = setup(prep, prec, use_graph=True)
+        f.start(steps)
+        f.step(steps)
+        irs[prec] = f.receivers(True)
+        f.close()
+    ref = irs.pop("fp64")
+    out = {"workload": f"random room 512x384x256, {steps} steps, {len(ref)} receivers, vs its FP64 run"}
+    for k, ir in irs.items():
+        out[k] = ir_errors(ir, ref)
     return out
 
 
@@ -254,11 +309,12 @@ def large_room(steps=1000, warmup=20, adj_steps=40000, ncp=200, adjoint=True):
           file=sys.stderr, flush=True)
     out = {"workload": "shoebox generator 1536x1024x768, 5 materials, 8 receivers, FP16 storage",
            "cells": n, "boundary_cells": int(len(prep.room.bidx))}
-    f = setup(prep, "fp16")
-    f.start(warmup + steps)
+    f = setup(prep, "fp16", use_graph=True)
+    f.start(warmup + SAMPLE_STEPS + steps)
     f.step(warmup)
     f.synchronize()
-    total_ms, dense_ms, bnd_ms = f.time_steps(steps)
+    tm = f.time_steps(steps, SAMPLE_STEPS)
+    total_ms, dense_ms, bnd_ms = tm["total_ms"], tm["dense_ms"], tm["boundary_ms"]
     db = dense_bytes_of(n, BYTES["fp16"])
     out.update({"steps": steps, "value": round(n * steps / (total_ms / 1e3) / 1e9, 3),
                 "unit": "Gvoxel-updates/s", "ms_per_step": round(total_ms / steps, 4),
@@ -375,6 +431,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost measurement")
     ap.add_argument("--no-precision", action="store_true", help="skip the storage-precision error runs")
     ap.add_argument("--no-images", action="store_true", help="skip the layer-image loop measurement")
+    ap.add_argument("--error-steps", type=int, default=20000,
+                    help="steps of the benchmark room's FP32/FP16/BF16 runs against FP64 (0 = skip)")
     ap.add_argument("--large", action="store_true",
                     help="only the 1536x1024x768 FP16 room and the FP32 adjoint on its 512^3 crop")
     ap.add_argument("--adj-steps", type=int, default=40000, help="steps of the --large crop adjoint")
@@ -436,7 +494,7 @@ def main():
         "config": {"workload": "random room 512x384x256 (6 boxes + 4 tilted slabs, 3 biquads, "
                                "8 receivers, Gaussian-derivative source)",
                    "cells": n, "boundary_cells": nb, "storage": args.precision,
-                   "launch": "individual launches" if args.no_graph else "30-step HIP graphs"},
+                   "launch": launch_label(r["timing"])},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(args.precision) if not args.fuse and not args.dense_blocks else None,
@@ -472,6 +530,8 @@ def main():
                           "dense_ms": round(rv["dense_ms"], 5), "boundary_kernel_ms": round(rv["bnd_ms"], 5)}
     if variants:
         out["variants"] = variants
+    if args.error_steps > 0:
+        out["error_vs_fp64"] = room_error_vs_fp64(prep, args.error_steps)
     if not args.no_precision:
         out["precision_error"] = precision_error()
     if not args.no_adjoint:

@@ -129,10 +129,16 @@ int virs_set_early_stop(virs_ctx *ctx, int32_t enable, float target_rms);
  * buffer for nsteps, zero fields, filter state and step counter. */
 int virs_start_simulation(virs_ctx *ctx, int64_t nsteps);
 /* nsteps calls of fdtd_gpu_step (simulator_gpu.cu:184-332), launched without
- * host round trips.  *keep_going (may be NULL) is 0 once the RMS rule fired;
- * the steps after it are not counted in virs_steps_done. */
+ * host round trips: with use_graph as replays of a 30-step HIP graph and one
+ * graph of the remaining steps, otherwise as single launches.  *keep_going
+ * (may be NULL) is 0 once the RMS rule fired; the steps after it are not
+ * counted in virs_steps_done.  VIRS_ERR_STATE after a virs_adjoint that
+ * recomputed segments, until virs_start_simulation. */
 int virs_step(virs_ctx *ctx, int64_t nsteps, int32_t *keep_going);
 int64_t virs_steps_done(const virs_ctx *ctx);
+/* Waits for the context's stream.  VIRS_ERR_STATE once the run diverged: a
+ * receiver sample or a corrected boundary value was not finite (every
+ * precision), or the 16-bit scale estimate was (FP16/BF16). */
 int virs_synchronize(virs_ctx *ctx);
 /* Grid::percent_to_target_RMS of the last step (vectorspace.h:98). */
 float virs_percent_to_target_rms(const virs_ctx *ctx);
@@ -165,15 +171,28 @@ int virs_field_exponent(virs_ctx *ctx, int32_t which, int32_t *e);
 /* Device bytes held by the context. */
 int virs_memory_bytes(const virs_ctx *ctx, int64_t *bytes);
 
-/* Runs nsteps and times them on the context's stream with HIP events:
- * total_ms over the whole run, and the mean duration of the stencil launch
- * and of the boundary/point launch over a sample of the steps (with
- * use_graph: every 31st step, the others replayed as 30-step graphs;
- * otherwise about 64 steps spread over the run).  A sampled launch records
- * its own start and stop (hipExtLaunchKernel), so the durations exclude the
- * dispatch around the kernels. */
-int virs_time_steps(virs_ctx *ctx, int64_t nsteps, double *total_ms, double *dense_ms,
-                    double *boundary_ms);
+/* Timing of a batch of steps (virs_time_steps).  No reference counterpart:
+ * the reference's loop is timed by the caller around fdtd_step calls. */
+typedef struct {
+    double total_ms;          /* HIP events around the timed region on the context's stream */
+    double dense_ms;          /* mean stencil launch over the sampling pass */
+    double boundary_ms;       /* mean boundary/point launch over the sampling pass */
+    int64_t sampled_steps;    /* steps of the sampling pass (before the timed region) */
+    int64_t graph_launches;   /* graph replays in the timed region */
+    int32_t graph_steps;      /* steps of each full graph (0: none replayed) */
+    int32_t tail_graph_steps; /* steps of the one shorter graph at the end (0: none) */
+    int64_t single_steps;     /* steps launched one by one in the timed region */
+} virs_timing;
+
+/* Runs nsample + nsteps steps.  First a sampling pass of nsample single
+ * steps whose two launches each record their own start and stop
+ * (hipExtLaunchKernel: the kernel's execution, without the dispatch around
+ * it), untimed as a whole; then the timed region: nsteps steps exactly as
+ * virs_step runs them (with use_graph: replays of a 30-step graph and one
+ * graph of the remaining steps, all built before the region starts;
+ * otherwise single launches), between two HIP events and nothing else.
+ * The RMS stop is off meanwhile. */
+int virs_time_steps(virs_ctx *ctx, int64_t nsteps, int64_t nsample, virs_timing *out);
 
 /* Adjoint gradient of an IR loss with respect to every material's
  * (b0,b1,b2,a1,a2); no reference counterpart.
@@ -183,10 +202,17 @@ int virs_time_steps(virs_ctx *ctx, int64_t nsteps, double *total_ms, double *den
  * and step) is recorded and nothing is recomputed; ncheckpoints = 0 picks
  * that when the trace takes at most a quarter of the free device memory,
  * else ceil(sqrt(nsteps)) checkpoints.
- * virs_adjoint takes dL/dIR [nrcv][nsteps] and writes grad[nmat][5]; it
+ * Changing the materials, flags, sources or boundary list afterwards
+ * drops the recording (virs_adjoint then fails with VIRS_ERR_STATE).
+ * virs_adjoint takes dL/dIR [nrcv][nsteps] (nrcv * nsteps floats, nsteps
+ * as recorded) and writes grad[nmat][5]; it
  * recomputes each segment forward from its snapshot and runs the
  * time-reversed stencil with the transposed boundary filters.  The gradient
- * does not depend on the number of checkpoints (bit for bit).  FP32/FP64. */
+ * does not depend on the number of checkpoints (bit for bit).  FP32/FP64.
+ * With two or more checkpoints the recomputation leaves the fields mid-run:
+ * virs_step, the field reads and virs_plane_request then fail with
+ * VIRS_ERR_STATE until virs_start_simulation (or virs_forward_record); the
+ * recorded IR stays readable. */
 int virs_forward_record(virs_ctx *ctx, int64_t nsteps, int32_t ncheckpoints);
 int virs_adjoint(virs_ctx *ctx, const float *dl_dir, double *grad);
 

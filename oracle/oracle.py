@@ -15,7 +15,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_SO = os.path.join(_HERE, "_build", "libvirs_oracle.so")
+# VIRS_ORACLE_SO selects another build of the oracle (the sanitized one)
+_SO = os.environ.get("VIRS_ORACLE_SO") or os.path.join(_HERE, "_build", "libvirs_oracle.so")
 _lib = None
 
 f32 = np.float32

@@ -34,3 +34,13 @@ def test_targets_report_each_goal_and_skip_missing_legs():
     assert t2 == {}
     t3 = b.targets({"roofline": {"frac": 0.6}}, "fp32")
     assert set(t3) == {"fp32_stencil_hbm_frac"} and t3["fp32_stencil_hbm_frac"]["met"] is False
+
+
+def test_launch_label_says_what_ran():
+    b = _bench()
+    base = {"graph_launches": 0, "graph_steps": 0, "tail_graph_steps": 0, "single_steps": 0}
+    assert b.launch_label(dict(base, graph_launches=1, tail_graph_steps=20)) == "one 20-step HIP graph"
+    assert (b.launch_label(dict(base, graph_launches=17, graph_steps=30, tail_graph_steps=20))
+            == "16 replays of a 30-step HIP graph + one 20-step HIP graph")
+    assert b.launch_label(dict(base, graph_launches=3, graph_steps=30)) == "3 replays of a 30-step HIP graph"
+    assert b.launch_label(dict(base, single_steps=20)) == "20 single-launch steps"

@@ -84,8 +84,9 @@ def test_adjoint_after_the_boundary_list_grows(gpu):
     """Record and adjoint on every other list entry, then the whole list
     uploaded into the same context: the checkpointed filter states, traces
     and reverse filter states follow the new length, the stale recording is
-    refused, and the new gradient equals a fresh context's bit for bit.  A
-    ninth material after recording is refused too."""
+    refused, and the new gradient equals a fresh context's bit for bit.  New
+    materials drop the recording, and a ninth material is refused by the
+    recording forward."""
     from virs_amd.fdtd import FDTDError
     p = tiny_problem(26, dims=(20, 16, 12), steps=60, sponge=2)
     n = p["steps"]
@@ -110,8 +111,11 @@ def test_adjoint_after_the_boundary_list_grows(gpu):
     assert not np.array_equal(g, g_half)
     f.forward_record(n, 4)
     f.set_materials(np.tile(p["coeffs"][:1], (9, 1)))
-    with pytest.raises(FDTDError, match="up to 8 materials"):
+    with pytest.raises(FDTDError, match="forward_record first"):
         f.adjoint(dl)
+    f.start(n)
+    with pytest.raises(FDTDError, match="up to 8 materials"):
+        f.forward_record(n, 4)
     f.close()
     fresh.close()
 

@@ -1,0 +1,151 @@
+"""Library rules on the GPU, through the C ABI:
+- a recorded run is dropped when what it computed changes (materials, flags,
+  sources), so the adjoint cannot mix old and new;
+- after an adjoint that recomputed segments, stepping and field reads wait
+  for a reset, while the recorded IR stays readable;
+- dL/dIR must have the recorded shape;
+- a diverging run is reported by virs_synchronize in every precision;
+- virs_time_steps times exactly the launches virs_step makes (graphs of any
+  length), after an untimed sampling pass."""
+import numpy as np
+import pytest
+
+from helpers import gpu_context, tiny_problem
+from virs_amd.fdtd import FDTDError
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("change", ["materials", "flags", "soft", "hard"])
+def test_changes_after_recording_drop_it(gpu, change):
+    p = tiny_problem(31, dims=(20, 16, 12), steps=40, sponge=2)
+    n = p["steps"]
+    _say(f"recording dropped by {change}")
+    f = gpu_context(p, "fp64")
+    f.forward_record(n, 3)
+    dl = np.zeros((len(p["rcv_idx"]), n), np.float32)
+    if change == "materials":
+        f.set_materials(p["coeffs"][::-1].copy())  # same count
+    elif change == "flags":
+        f.upload_flags(p["flags"])
+    elif change == "soft":
+        f.set_soft_sources(p["soft_idx"], p["soft_sig"])
+    else:
+        f.set_hard_source(None)
+    with pytest.raises(FDTDError, match="forward_record first"):
+        f.adjoint(dl)
+    # a new recording works again
+    f.start(n)
+    f.forward_record(n, 3)
+    assert np.isfinite(f.adjoint(dl)).all()
+    f.close()
+
+
+@pytest.mark.parametrize("ncp", [1, 3])
+def test_stepping_after_adjoint(gpu, ncp):
+    p = tiny_problem(32, dims=(20, 16, 12), steps=45, sponge=2)
+    n = p["steps"]
+    _say(f"stepping after an adjoint with {ncp} checkpoints")
+    f = gpu_context(p, "fp32")
+    f.forward_record(n, ncp)
+    ir = f.receivers(True)
+    end = f.field(0)
+    dl = np.random.default_rng(3).standard_normal(ir.shape).astype(np.float32)
+    f.adjoint(dl)
+    np.testing.assert_array_equal(f.receivers(True), ir)
+    if ncp == 1:
+        # nothing was recomputed: the finished run's state is still there
+        np.testing.assert_array_equal(f.field(0), end)
+    else:
+        with pytest.raises(FDTDError, match="virs_start_simulation"):
+            f.step(1)
+        with pytest.raises(FDTDError, match="virs_start_simulation"):
+            f.field(0)
+        with pytest.raises(FDTDError, match="virs_start_simulation"):
+            f.plane_request(1)
+    f.start(n)
+    f.step(n)
+    np.testing.assert_array_equal(f.receivers(True), ir)
+    np.testing.assert_array_equal(f.field(0), end)
+    f.close()
+
+
+def test_dl_dir_shape_is_checked(gpu):
+    p = tiny_problem(33, dims=(20, 16, 12), steps=30, sponge=0)
+    n, r = p["steps"], len(p["rcv_idx"])
+    f = gpu_context(p, "fp32")
+    f.forward_record(n, 2)
+    for bad in ((r, n - 1), (n, r), (r - 1, n)):
+        with pytest.raises(ValueError, match="dl_dir"):
+            f.adjoint(np.zeros(bad, np.float32))
+    assert np.isfinite(f.adjoint(np.ones((r, n), np.float32))).all()
+    f.close()
+
+
+@pytest.mark.parametrize("case", ["fp32-biquad", "fp64-biquad", "fp16-biquad", "fp32-courant"])
+def test_divergence_is_reported(gpu, case):
+    """An unstable biquad (poles at 2.32 and 1.68) or a Courant number above
+    the 3-D limit (lambda^2 = 0.5) overflows the fields to Inf/NaN; nothing
+    faults, and virs_synchronize reports the divergence."""
+    prec, kind = case.split("-")
+    if kind == "biquad":
+        p = tiny_problem(34, dims=(20, 16, 12), steps=1500, sponge=0, hard=False)
+        p["coeffs"] = np.tile([1.0, 0.0, 0.0, -4.0, 3.9], (len(p["coeffs"]), 1))
+    else:
+        p = tiny_problem(35, dims=(20, 16, 12), steps=400, sponge=0, hard=False, lam2=0.5)
+    _say(f"divergence {case}, {p['steps']} steps")
+    f = gpu_context(p, prec)
+    f.step(p["steps"])
+    with pytest.raises(FDTDError, match="diverged"):
+        f.synchronize()
+    # a fresh run clears the flag
+    f.start(p["steps"])
+    f.synchronize()
+    f.close()
+
+
+def test_stable_run_is_not_flagged(gpu):
+    p = tiny_problem(36, dims=(20, 16, 12), steps=300, sponge=2)
+    f = gpu_context(p, "fp32")
+    f.step(p["steps"])
+    f.synchronize()
+    f.close()
+
+
+@pytest.mark.parametrize("n", [20, 47, 65])
+def test_timed_steps_are_virs_steps(gpu, n):
+    """virs_time_steps: nsample single steps, then n steps as graph replays
+    (full 30-step graphs and one graph of the rest); the result equals plain
+    single-launch stepping bit for bit, and virs_step with graphs as well."""
+    nsample = 8
+    p = tiny_problem(37, dims=(24, 18, 14), steps=nsample + n + 5, sponge=3)
+    _say(f"timed steps n={n}")
+    a = gpu_context(p, "fp32", use_graph=True)
+    t = a.time_steps(n, nsample)
+    full, tail = divmod(n, 30)
+    assert t["sampled_steps"] == nsample
+    assert t["graph_launches"] == full + (tail > 0)
+    assert t["graph_steps"] == (30 if full else 0)
+    assert t["tail_graph_steps"] == tail
+    assert t["single_steps"] == 0
+    assert t["total_ms"] > 0 and t["dense_ms"] > 0 and t["boundary_ms"] > 0
+    b = gpu_context(p, "fp32", use_graph=False)
+    b.step(nsample + n)
+    c = gpu_context(p, "fp32", use_graph=True)
+    c.step(nsample)
+    c.step(n)
+    assert a.steps_done == b.steps_done == c.steps_done == nsample + n
+    for other in (b, c):
+        np.testing.assert_array_equal(a.receivers(True), other.receivers(True))
+        np.testing.assert_array_equal(a.field(0), other.field(0))
+        np.testing.assert_array_equal(a.field(1), other.field(1))
+    u = gpu_context(p, "fp32", use_graph=False)
+    tu = u.time_steps(n, nsample)
+    assert tu["single_steps"] == n and tu["graph_launches"] == 0
+    np.testing.assert_array_equal(u.receivers(True), b.receivers(True))
+    for f in (a, b, c, u):
+        f.close()

@@ -36,6 +36,15 @@ class VirsDesc(C.Structure):
     ]
 
 
+class VirsTiming(C.Structure):
+    """virs_timing of include/virs_fdtd.h."""
+    _fields_ = [
+        ("total_ms", C.c_double), ("dense_ms", C.c_double), ("boundary_ms", C.c_double),
+        ("sampled_steps", C.c_int64), ("graph_launches", C.c_int64), ("graph_steps", C.c_int32),
+        ("tail_graph_steps", C.c_int32), ("single_steps", C.c_int64),
+    ]
+
+
 def _load(name):
     path = os.path.join(LIB_DIR, name)
     if not os.path.exists(path):
@@ -84,7 +93,7 @@ def hip():
     _sig(L, "virs_write_field_f64", C.c_int, [vp, C.c_int32, c_dp])
     _sig(L, "virs_field_exponent", C.c_int, [vp, C.c_int32, c_i32p])
     _sig(L, "virs_memory_bytes", C.c_int, [vp, c_i64p])
-    _sig(L, "virs_time_steps", C.c_int, [vp, C.c_int64, c_dp, c_dp, c_dp])
+    _sig(L, "virs_time_steps", C.c_int, [vp, C.c_int64, C.c_int64, C.POINTER(VirsTiming)])
     _sig(L, "virs_forward_record", C.c_int, [vp, C.c_int64, C.c_int32])
     _sig(L, "virs_adjoint", C.c_int, [vp, c_fp, c_dp])
     _sig(L, "virs_plane_request", C.c_int, [vp, C.c_int64, c_i32p])

@@ -8,6 +8,11 @@ produces
 and, for the tests, the dispatch adapter (include/virs_dispatch.hpp) compiled
 against a stand-in VectorSpace:
     tests/adapter/_build/libdispatch_driver.so
+
+    python -m virs_amd.build --sanitize DIR
+
+builds the host library with AddressSanitizer and UndefinedBehaviorSanitizer
+(host code only) into DIR, next to an ordinary build of the HIP library.
 """
 import os
 import subprocess
@@ -29,6 +34,7 @@ HIP_FLAGS = [
     f"-I{INC}",
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-pthread", f"-I{INC}"]
+SANITIZE_FLAGS = ["-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-fno-omit-frame-pointer", "-g"]
 
 
 def _run(cmd, verbose):
@@ -40,16 +46,20 @@ def _run(cmd, verbose):
         raise RuntimeError(f"build failed: {' '.join(cmd)}")
 
 
-def build(verbose=False, out_dir=None, defines=()):
-    """out_dir / defines build a variant (e.g. VIRS_TAIL_BATCH=2) elsewhere."""
+def build(verbose=False, out_dir=None, defines=(), sanitize=False):
+    """out_dir / defines build a variant (e.g. VIRS_TAIL_BATCH=2) elsewhere;
+    sanitize builds only the host library, with ASan + UBSan."""
     lib = out_dir or LIB
     os.makedirs(lib, exist_ok=True)
     hip_src = os.path.join(PKG, "csrc", "fdtd", "virs_fdtd.hip")
     host_src = os.path.join(PKG, "csrc", "host", "virs_host.cpp")
     hip_out = os.path.join(lib, "libvirs_hip.so")
     host_out = os.path.join(lib, "libvirs_host.so")
-    _run(["g++"] + HOST_FLAGS + [host_src, "-o", host_out + ".tmp", "-lz"], verbose)
+    extra = SANITIZE_FLAGS if sanitize else []
+    _run(["g++"] + HOST_FLAGS + extra + [host_src, "-o", host_out + ".tmp", "-lz"], verbose)
     os.replace(host_out + ".tmp", host_out)
+    if sanitize:  # host code only; the HIP library is built the ordinary way
+        return None, host_out
     _run([HIPCC] + HIP_FLAGS + [f"-D{d}" for d in defines] + [hip_src, "-o", hip_out + ".tmp"], verbose)
     os.replace(hip_out + ".tmp", hip_out)
     return hip_out, host_out
@@ -67,13 +77,9 @@ def build_dispatch_driver(verbose=False):
     return out
 
 
-def build_oracle(verbose=False):
-    cmd = ["make", "-s", "-C", os.path.join(ROOT, "oracle")]
-    _run(cmd, verbose)
-    return os.path.join(ROOT, "oracle", "_build", "libvirs_oracle.so")
-
-
 if __name__ == "__main__":
-    build(verbose=True)
-    build_dispatch_driver(verbose=True)
-    build_oracle(verbose=True)
+    if len(sys.argv) == 3 and sys.argv[1] == "--sanitize":
+        build(verbose=True, out_dir=sys.argv[2], sanitize=True)
+    else:
+        build(verbose=True)
+        build_dispatch_driver(verbose=True)

@@ -236,6 +236,15 @@ __device__ inline void vstore(VT *p, const VT &v)
 
 __device__ inline float exp2i(int e) { return __int_as_float((e + 127) << 23); }
 
+// Divergence check of the list and point launches: a non-finite corrected
+// boundary value or receiver sample marks the run (StepCtl::err), which
+// virs_synchronize reports in every precision.
+template <typename C>
+__device__ inline void flag_nonfinite(StepCtl *ctl, C v)
+{
+    if (!__builtin_isfinite(v)) ctl->err = 1;
+}
+
 template <typename C>
 __device__ inline C wave_max(C v)
 {
@@ -1187,6 +1196,7 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
             r = corrected<C>(r, p.bs.kconst, yv, in);
             T t = store_round<T>(r * sR, p.sr, (unsigned long long)b, n, kSaltBoundary);
             if (e < nb && !skip) {
+                flag_nonfinite(p.ctl, r);
                 store_taps<C, Y>(p.bs, e, n, dif, yv);
                 // a soft source on this cell is added to the stored corrected
                 // value (the point tables are scanned only when some point is
@@ -1205,6 +1215,7 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
                         if (p.rcv_bpos[k] == e) {
                             C stored = S::ld(t);
                             if constexpr (S::kScaled) stored = stored * sRi;
+                            flag_nonfinite(p.ctl, stored);
                             p.ir[(long long)k * p.ir_cap + n] = (double)stored;
                         }
             }
@@ -1234,6 +1245,7 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
             if (in_ir) {
                 C v = S::ld(t);
                 if constexpr (S::kScaled) v = v * sRi;
+                if (j0 >= 0) flag_nonfinite(ctl, v);
                 for (int j = j0; j >= 0; j = p.rcv_next[j]) p.ir[(long long)j * p.ir_cap + n] = (double)v;
             }
         }
@@ -1243,7 +1255,10 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
         const bool by_source = in_sig && p.rcv_src[kr] >= 0;
         C r = S::ld(R[i]);
         if constexpr (S::kScaled) r = r * sRi;
-        if (bp < 0 && !by_source) p.ir[(long long)kr * p.ir_cap + n] = (double)r;
+        if (bp < 0 && !by_source) {
+            flag_nonfinite(ctl, r);
+            p.ir[(long long)kr * p.ir_cap + n] = (double)r;
+        }
     }
     // RMS rule of step n over receiver 0's samples [n-100, n), in float in
     // the reference's order (simulator_gpu.cu:303-330)

@@ -87,8 +87,12 @@ struct virs_ctx {
     int es_enable = 0;
     float es_target = 0.0009f, percent = 0.f;
 
+    // graphs by starting rotation phase: kGraphSteps steps each, and one
+    // shorter graph per phase for the remainder of a batch (tail_steps[k]
+    // steps)
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
-    int graph_steps = 0;
+    hipGraphExec_t tail_graph[3] = {nullptr, nullptr, nullptr};
+    int tail_steps[3] = {0, 0, 0};
     // when set, the next stencil or list launch records its own start and
     // stop in these events (hipExtLaunchKernel: the kernel's execution, not
     // the dispatch around it); launch_step sets them for sampled steps
@@ -96,7 +100,14 @@ struct virs_ctx {
 
     // adjoint
     int ncp = 0;
+    // steps of the run recorded by virs_forward_record; 0 when there is none
+    // or when a later call (materials, flags, sources, boundary list) changed
+    // what the recorded run computed
     long long rec_steps = 0, seg = 0;
+    // set by virs_adjoint when its recomputation left the fields, filter
+    // state and step counter mid-run: stepping and field reads are refused
+    // until virs_start_simulation or virs_forward_record resets the run
+    bool restart_needed = false;
     std::vector<void *> cp_field;   // [ncp][2] stored fields
     std::vector<uint8_t *> cp_state; // [ncp] the four state rings back to back
     // two boundary-trace buffers (segment k uses buffer k & 1), elements of
@@ -170,13 +181,17 @@ void dfree(P *&p)
 
 int grid_for(long long n) { return (int)std::min<long long>(std::max<long long>((n + 255) / 256, 1), 65536); }
 
+// Steps per replayed graph; a multiple of 3 keeps the buffer rotation phase.
+constexpr int kGraphSteps = 30;
+
 void drop_graphs(virs_ctx *c)
 {
-    for (auto &g : c->graph)
-        if (g) {
-            (void)hipGraphExecDestroy(g);
-            g = nullptr;
-        }
+    for (int k = 0; k < 3; ++k) {
+        if (c->graph[k]) (void)hipGraphExecDestroy(c->graph[k]);
+        if (c->tail_graph[k]) (void)hipGraphExecDestroy(c->tail_graph[k]);
+        c->graph[k] = c->tail_graph[k] = nullptr;
+        c->tail_steps[k] = 0;
+    }
 }
 
 // Position of linear index `idx` in the sorted boundary list, or -1.
@@ -605,7 +620,19 @@ int reset_run(virs_ctx *c)
     c->steps = c->counted = 0;
     c->stopped = false;
     c->percent = 0.f;
+    c->restart_needed = false;
     drop_graphs(c);
+    return VIRS_OK;
+}
+
+// Stepping and field reads after virs_adjoint, whose recomputation leaves
+// the fields mid-run, need a reset first.
+int check_not_mid_adjoint(virs_ctx *c)
+{
+    if (c->restart_needed)
+        return fail(c, VIRS_ERR_STATE,
+                    "virs_adjoint left the fields mid-run: call virs_start_simulation (or "
+                    "virs_forward_record) before stepping or reading fields");
     return VIRS_OK;
 }
 
@@ -844,6 +871,7 @@ int virs_upload_flags(virs_ctx *c, const uint8_t *flags)
             }
     rc = check_soft_cells(c, c->h_soft_idx.data(), c->nsoft);
     if (rc) return rc;
+    c->rec_steps = 0; // a recorded run used the old cell codes
     drop_graphs(c);
     return refresh_fusion(c);
 }
@@ -923,6 +951,7 @@ int virs_set_materials(virs_ctx *c, const double *coeffs, int32_t nmat)
     if (rc) return rc;
     HIPC(c, hipMemcpy(c->coeffs, coeffs, sizeof(double) * 5 * nmat, hipMemcpyHostToDevice));
     c->nmat = nmat;
+    c->rec_steps = 0; // a recorded run used the old coefficients
     drop_graphs(c);
     return VIRS_OK;
 }
@@ -940,6 +969,7 @@ int virs_set_hard_source(virs_ctx *c, const double *sig, int64_t len)
         if (rc) return rc;
         HIPC(c, hipMemcpy(c->hard_sig, sig, sizeof(double) * len, hipMemcpyHostToDevice));
     }
+    c->rec_steps = 0; // a recorded run used the old source
     drop_graphs(c);
     return VIRS_OK;
 }
@@ -970,6 +1000,7 @@ int virs_set_soft_sources(virs_ctx *c, const int64_t *idx, const double *sig, in
         if (rc) return rc;
         HIPC(c, hipMemcpy(c->soft_sig, sig, sizeof(double) * nsrc * len, hipMemcpyHostToDevice));
     }
+    c->rec_steps = 0; // a recorded run used the old sources
     drop_graphs(c);
     return upload_points(c);
 }
@@ -1026,9 +1057,9 @@ int virs_start_simulation(virs_ctx *c, int64_t nsteps)
     return VIRS_OK;
 }
 
-static int build_graph(virs_ctx *c, int steps)
+// Captures `steps` plain steps from the current rotation phase into *out.
+static int build_graph(virs_ctx *c, int steps, hipGraphExec_t *out)
 {
-    const int phase = c->ip;
     const int sp = c->ip, sq = c->iq, sr = c->ir;
     hipGraph_t g = nullptr;
     HIPC(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1043,32 +1074,78 @@ static int build_graph(virs_ctx *c, int steps)
         return rc;
     }
     if (e != hipSuccess) return fail(c, VIRS_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
-    e = hipGraphInstantiate(&c->graph[phase], g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) {
-        c->graph[phase] = nullptr;
+        *out = nullptr;
         return fail(c, VIRS_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
     }
-    c->graph_steps = steps;
     return VIRS_OK;
 }
 
-// n plain steps (receivers on, no trace) on the context's stream: replays of
-// the 30-step graph of the current rotation phase when the context uses
-// graphs, single launches for the rest.
+// How run_steps splits n steps when the context uses graphs: `full`
+// replays of the kGraphSteps-step graph, then one graph of `tail` steps.
+static void graph_plan(long long n, long long &full, int &tail)
+{
+    full = n / kGraphSteps;
+    tail = (int)(n % kGraphSteps);
+}
+
+// Makes sure the graphs that run_steps(n) replays from the current rotation
+// phase exist (whole replays keep the phase, so the tail starts from it too).
+static int ensure_graphs_for(virs_ctx *c, long long n)
+{
+    long long full;
+    int tail;
+    graph_plan(n, full, tail);
+    const int ph = c->ip;
+    if (full > 0 && !c->graph[ph]) {
+        const int rc = build_graph(c, kGraphSteps, &c->graph[ph]);
+        if (rc) return rc;
+    }
+    if (tail > 0 && (!c->tail_graph[ph] || c->tail_steps[ph] != tail)) {
+        if (c->tail_graph[ph]) (void)hipGraphExecDestroy(c->tail_graph[ph]);
+        c->tail_graph[ph] = nullptr;
+        c->tail_steps[ph] = 0;
+        const int rc = build_graph(c, tail, &c->tail_graph[ph]);
+        if (rc) return rc;
+        c->tail_steps[ph] = tail;
+    }
+    return VIRS_OK;
+}
+
+// Applies the buffer rotation of k steps to the host's indices.
+static void rotate(virs_ctx *c, long long k)
+{
+    for (long long j = 0; j < k % 3; ++j) {
+        const int ip = c->ip;
+        c->ip = c->ir;
+        c->ir = c->iq;
+        c->iq = ip;
+    }
+}
+
+// n plain steps (receivers on, no trace) on the context's stream.  With
+// graphs: replays of the kGraphSteps-step graph of the current rotation
+// phase, then one graph of the remaining steps; otherwise single launches.
 static int run_steps(virs_ctx *c, long long n)
 {
-    const int gsteps = 30; // a multiple of 3 keeps the buffer rotation phase
-    if (c->d.use_graph) {
-        while (n >= gsteps) {
-            if (!c->graph[c->ip] || c->graph_steps != gsteps) {
-                const int rc = build_graph(c, gsteps);
-                if (rc) return rc;
-            }
+    if (c->d.use_graph && n > 0) {
+        int rc = ensure_graphs_for(c, n);
+        if (rc) return rc;
+        long long full;
+        int tail;
+        graph_plan(n, full, tail);
+        for (long long k = 0; k < full; ++k) {
             HIPC(c, hipGraphLaunch(c->graph[c->ip], c->stream));
-            n -= gsteps;
-            c->steps += gsteps;
+            c->steps += kGraphSteps;
         }
+        if (tail > 0) {
+            HIPC(c, hipGraphLaunch(c->tail_graph[c->ip], c->stream));
+            c->steps += tail;
+            rotate(c, tail);
+        }
+        return VIRS_OK;
     }
     for (; n > 0; --n) {
         const int rc = launch_step(c);
@@ -1081,6 +1158,7 @@ static int run_steps(virs_ctx *c, long long n)
 int virs_step(virs_ctx *c, int64_t nsteps, int32_t *keep_going)
 {
     int rc = check_ready(c);
+    if (!rc) rc = check_not_mid_adjoint(c);
     if (rc) return rc;
     if (nsteps < 0) return VIRS_ERR_ARG;
     if (keep_going) *keep_going = c->stopped ? 0 : 1;
@@ -1132,7 +1210,10 @@ int virs_synchronize(virs_ctx *c)
     HIPC(c, hipStreamSynchronize(c->stream));
     int err = 0;
     HIPC(c, hipMemcpy(&err, &c->ctl->err, sizeof(int), hipMemcpyDeviceToHost));
-    if (err) return fail(c, VIRS_ERR_STATE, "non-finite field maximum: the run diverged");
+    if (err)
+        return fail(c, VIRS_ERR_STATE,
+                    "the run diverged: a receiver sample, a corrected boundary value or the 16-bit "
+                    "scale estimate was not finite");
     return VIRS_OK;
 }
 
@@ -1163,6 +1244,7 @@ int virs_field_exponent(virs_ctx *c, int32_t which, int32_t *e)
 {
     if (!c || !e || which < 0 || which > 2) return VIRS_ERR_ARG;
     *e = 0;
+    if (int rc = check_not_mid_adjoint(c)) return rc;
     if (c->d.precision != VIRS_FP16 && c->d.precision != VIRS_BF16) return VIRS_OK;
     HIPC(c, hipSetDevice(c->d.device));
     HIPC(c, hipStreamSynchronize(c->stream));
@@ -1205,6 +1287,7 @@ int virs_read_field_f64(virs_ctx *c, int32_t which, double *out)
 int virs_read_field(virs_ctx *c, int32_t which, float *out)
 {
     if (!c || !out || which < 0 || which > 2) return VIRS_ERR_ARG;
+    if (int rc = check_not_mid_adjoint(c)) return rc;
     if (c->d.precision == VIRS_FP32) {
         HIPC(c, hipSetDevice(c->d.device));
         HIPC(c, hipMemcpyAsync(out, field_buffer(c, which), sizeof(float) * c->N, hipMemcpyDeviceToHost,
@@ -1310,6 +1393,7 @@ int virs_plane_request(virs_ctx *c, int64_t layer, int32_t *ticket)
     if (!c || !ticket) return VIRS_ERR_ARG;
     if (layer < 0 || layer >= c->ny)
         return fail(c, VIRS_ERR_ARG, "layer %lld out of range (Ny = %lld)", (long long)layer, c->ny);
+    if (int rc = check_not_mid_adjoint(c)) return rc;
     HIPC(c, hipSetDevice(c->d.device));
     int rc = ensure_snapshots(c);
     if (rc) return rc;
@@ -1385,94 +1469,66 @@ int virs_memory_bytes(const virs_ctx *c, int64_t *bytes)
     return VIRS_OK;
 }
 
-// Builds the graphs of all three rotation phases that are missing.
-static int ensure_graphs(virs_ctx *c, int steps)
-{
-    const int sp = c->ip, sq = c->iq, sr = c->ir;
-    int rc = VIRS_OK;
-    for (int k = 0; k < 3 && !rc; ++k) {
-        if (!c->graph[c->ip] || c->graph_steps != steps) rc = build_graph(c, steps);
-        const int ip = c->ip; // the rotation of one step, as launch_step applies it
-        c->ip = c->ir;
-        c->ir = c->iq;
-        c->iq = ip;
-    }
-    c->ip = sp;
-    c->iq = sq;
-    c->ir = sr;
-    return rc;
-}
-
-int virs_time_steps(virs_ctx *c, int64_t nsteps, double *total_ms, double *dense_ms, double *boundary_ms)
+int virs_time_steps(virs_ctx *c, int64_t nsteps, int64_t nsample, virs_timing *out)
 {
     int rc = check_ready(c);
+    if (!rc) rc = check_not_mid_adjoint(c);
     if (rc) return rc;
-    if (nsteps <= 0) return VIRS_ERR_ARG;
-    // Launch durations are taken on a sample of the steps: each of its two
-    // launches records its own start and stop (hipExtLaunchKernel), which
-    // excludes the dispatch around it.  With graphs, every sampled step is
-    // followed by one replay of a 30-step graph; otherwise about 64 steps
-    // spread over the run are sampled and the others launched plainly.
-    // Either way the sampling barely touches the loop time.
-    const int gsteps = 30;
-    const bool graphs = c->d.use_graph && nsteps > gsteps;
-    if (graphs) {
-        rc = ensure_graphs(c, gsteps);
-        if (rc) return rc;
-    }
-    rc = device_early_stop(c, false); // every timed step does its work
+    if (nsteps <= 0 || nsample < 0 || !out) return VIRS_ERR_ARG;
+    std::memset(out, 0, sizeof *out);
+    rc = device_early_stop(c, false); // every sampled or timed step does its work
     if (rc) return rc;
-    const long long stride = graphs ? gsteps + 1 : std::max<long long>(1, nsteps / 64);
-    const long long nsamp = (nsteps + stride - 1) / stride;
-    std::vector<hipEvent_t> ev((size_t)nsamp * 4);
+    // Sampling pass: nsample single steps whose two launches record their
+    // own start and stop (hipExtLaunchKernel), outside the timed region.
+    std::vector<hipEvent_t> ev((size_t)nsample * 4, nullptr);
     for (auto &e : ev) HIPC(c, hipEventCreate(&e));
-    hipEvent_t t0, t1;
-    HIPC(c, hipEventCreate(&t0));
-    HIPC(c, hipEventCreate(&t1));
-    HIPC(c, hipStreamSynchronize(c->stream));
-    HIPC(c, hipEventRecord(t0, c->stream));
-    long long s = 0, k = 0;
-    while (s < nsteps && !rc) {
+    for (long long k = 0; k < nsample && !rc; ++k) {
         rc = launch_step(c, nullptr, nullptr, 0, true, &ev[(size_t)k * 4]);
-        ++k;
-        ++s;
         ++c->steps;
-        long long plain = std::min<long long>(stride - 1, nsteps - s);
-        if (graphs)
-            while (!rc && plain >= gsteps) {
-                if (hipGraphLaunch(c->graph[c->ip], c->stream) != hipSuccess)
-                    rc = fail(c, VIRS_ERR_HIP, "graph launch failed");
-                plain -= gsteps;
-                s += gsteps;
-                c->steps += gsteps;
-            }
-        for (; plain > 0 && !rc; --plain) {
-            rc = launch_step(c);
-            ++s;
-            ++c->steps;
-        }
     }
+    // The timed region runs exactly what virs_step runs; its graphs are
+    // built before it starts.
+    if (!rc && c->d.use_graph) rc = ensure_graphs_for(c, nsteps);
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (!rc && hipEventCreate(&t0) != hipSuccess) rc = fail(c, VIRS_ERR_HIP, "event create");
+    if (!rc && hipEventCreate(&t1) != hipSuccess) rc = fail(c, VIRS_ERR_HIP, "event create");
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = fail(c, VIRS_ERR_HIP, "stream sync");
+    if (!rc && hipEventRecord(t0, c->stream) != hipSuccess) rc = fail(c, VIRS_ERR_HIP, "event record");
+    if (!rc) rc = run_steps(c, nsteps);
+    if (!rc && hipEventRecord(t1, c->stream) != hipSuccess) rc = fail(c, VIRS_ERR_HIP, "event record");
     c->counted = c->steps;
-    if (!rc) HIPC(c, hipEventRecord(t1, c->stream));
     if (!rc) rc = device_early_stop(c, true);
-    if (!rc) HIPC(c, hipEventSynchronize(t1));
+    if (!rc && hipEventSynchronize(t1) != hipSuccess) rc = fail(c, VIRS_ERR_HIP, "event sync");
     double dsum = 0, bsum = 0;
-    for (long long j = 0; j < k && !rc; ++j) {
+    for (long long j = 0; j < nsample && !rc; ++j) {
         float a = 0, b = 0;
-        HIPC(c, hipEventElapsedTime(&a, ev[j * 4], ev[j * 4 + 1]));
-        HIPC(c, hipEventElapsedTime(&b, ev[j * 4 + 2], ev[j * 4 + 3]));
+        if (hipEventElapsedTime(&a, ev[j * 4], ev[j * 4 + 1]) != hipSuccess ||
+            hipEventElapsedTime(&b, ev[j * 4 + 2], ev[j * 4 + 3]) != hipSuccess)
+            rc = fail(c, VIRS_ERR_HIP, "sampled launch events");
         dsum += a;
         bsum += b;
     }
     float tot = 0;
-    if (!rc) HIPC(c, hipEventElapsedTime(&tot, t0, t1));
-    for (auto &e : ev) (void)hipEventDestroy(e);
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
+    if (!rc && hipEventElapsedTime(&tot, t0, t1) != hipSuccess) rc = fail(c, VIRS_ERR_HIP, "timed region events");
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
     if (rc) return rc;
-    if (total_ms) *total_ms = tot;
-    if (dense_ms) *dense_ms = dsum / (double)k;
-    if (boundary_ms) *boundary_ms = bsum / (double)k;
+    out->total_ms = tot;
+    out->dense_ms = nsample ? dsum / (double)nsample : 0.0;
+    out->boundary_ms = nsample ? bsum / (double)nsample : 0.0;
+    out->sampled_steps = nsample;
+    if (c->d.use_graph) {
+        long long full;
+        int tail;
+        graph_plan(nsteps, full, tail);
+        out->graph_launches = full + (tail > 0);
+        out->graph_steps = full > 0 ? kGraphSteps : 0;
+        out->tail_graph_steps = tail;
+    } else {
+        out->single_steps = nsteps;
+    }
     return VIRS_OK;
 }
 
@@ -1751,9 +1807,13 @@ int virs_adjoint(virs_ctx *c, const float *dl_dir, double *grad)
     for (int i = 0; i < 5 * c->nmat; ++i) grad[i] = 0.0;
     for (long long b = 0; b < gb; ++b)
         for (int i = 0; i < 5 * c->nmat; ++i) grad[i] += part[(size_t)b * 5 * c->nmat + i];
-    // the forward state is left as after a finished run; a new
-    // virs_forward_record is needed before the next adjoint
+    // A new virs_forward_record is needed before the next adjoint.  With
+    // more than one segment the recomputation left the fields, filter state
+    // and step counter at the start of the second segment, so stepping and
+    // field reads wait for a reset; with one segment nothing was recomputed
+    // and the forward state is the finished run's.
     c->rec_steps = 0;
+    c->restart_needed = c->ncp >= 2;
     rc = device_early_stop(c, true);
     if (rc) return rc;
     HIPC(c, hipStreamSynchronize(c->stream));

@@ -221,20 +221,32 @@ class FDTD:
         self._chk(self.L.virs_memory_bytes(self._ctx, C.byref(b)), "memory")
         return b.value
 
-    def time_steps(self, nsteps):
-        """(total_ms, mean stencil-launch ms, mean boundary-launch ms) over nsteps."""
-        t, d, b = C.c_double(), C.c_double(), C.c_double()
-        self._chk(self.L.virs_time_steps(self._ctx, int(nsteps), C.byref(t), C.byref(d), C.byref(b)), "time steps")
-        return t.value, d.value, b.value
+    def time_steps(self, nsteps, nsample=32):
+        """Times nsteps steps as virs_step runs them, after a sampling pass of
+        nsample single steps that gives the mean launch durations.  Returns a
+        dict: total_ms (the timed region), dense_ms and boundary_ms (mean
+        launch durations over the sampling pass), and what the timed region
+        launched (graph_launches, graph_steps, tail_graph_steps,
+        single_steps)."""
+        t = _lib.VirsTiming()
+        self._chk(self.L.virs_time_steps(self._ctx, int(nsteps), int(nsample), C.byref(t)), "time steps")
+        return {k: getattr(t, k) for k, _ in t._fields_}
 
     # -- adjoint -----------------------------------------------------------
     def forward_record(self, nsteps, ncheckpoints):
+        self._rec_steps = 0
         self._chk(self.L.virs_forward_record(self._ctx, int(nsteps), int(ncheckpoints)), "forward record")
+        self._rec_steps = int(nsteps)
 
     def adjoint(self, dl_dir):
-        g = np.zeros((max(self.nmat, 1), 5), np.float64)
+        """dL/dIR [nrcv, recorded steps] -> gradient [nmat, 5]."""
         d = np.ascontiguousarray(np.asarray(dl_dir, np.float32))
+        want = (self.nrcv, getattr(self, "_rec_steps", 0))
+        if want[1] and d.shape != want:
+            raise ValueError(f"dl_dir has shape {d.shape}, expected {want} (receivers, recorded steps)")
+        g = np.zeros((max(self.nmat, 1), 5), np.float64)
         self._chk(self.L.virs_adjoint(self._ctx, d.ctypes.data_as(_lib.c_fp), g.ctypes.data_as(_lib.c_dp)), "adjoint")
+        self._rec_steps = 0
         return g[: self.nmat]
 
 
