@@ -156,3 +156,69 @@ def test_pressure_sphere_initial_condition(gpu):
     ref = pb.run(steps, "f32", want_fields=True)
     np.testing.assert_array_equal(out["ir"], ref["ir"][0].astype(np.float32))
     np.testing.assert_array_equal(out["p"], ref["p"].astype(np.float32))
+
+
+def _baffled_hall():
+    """Pre-shell flags of a 128x96x96 room (1.18 M cells): a walled hall
+    with x-normal baffles every 6 cells whose gaps alternate between the two
+    y ends (a maze that absorbs fast through its boundary cells), and one
+    hard-source voxel at (10, 48, 47)."""
+    nx, ny, nz = 128, 96, 96
+    f = np.zeros((nz, ny, nx), np.uint8)
+    f[2, 2:ny - 2, 2:nx - 2] = 1
+    f[nz - 3, 2:ny - 2, 2:nx - 2] = 1
+    f[2:nz - 2, 2, 2:nx - 2] = 1
+    f[2:nz - 2, ny - 3, 2:nx - 2] = 1
+    f[2:nz - 2, 2:ny - 2, 2] = 1
+    f[2:nz - 2, 2:ny - 2, nx - 3] = 1
+    for k, x in enumerate(range(16, nx - 8, 6)):
+        f[2:nz - 2, 2:ny - 2, x] = 1
+        if k % 2:
+            f[2:nz - 2, 2:12, x] = 0
+        else:
+            f[2:nz - 2, ny - 12:ny - 2, x] = 0
+    f[47, 48, 10] = 3
+    return f
+
+
+@pytest.mark.gpu
+def test_reference_default_scheme_with_rms_stop(gpu):
+    """The reference's own scheme end to end through the dispatch adapter:
+    its cell size h = 343/88200 and dt (lambda^2 = 1/12, simulator_gpu.cu:196),
+    its material row (RBJ high-pass 440 Hz, Q 0.1 at fs = 1/dt,
+    simulator.cpp:942-967; a real pole at 0.99909), the flag-3 hard pulse
+    (simulator_gpu.cu:214-218), one receiver and the live RMS stop
+    (simulator_gpu.cu:303-330), on a 1.18 M-cell room where the rule fires
+    well after step 3000 (the one-pole recursion runs for ~900 steps).
+    The GPU IR equals the FP64-state oracle bit for bit and is within 1e-4
+    relative L2 of the reference-literal oracle (float filter state, the
+    reference's arithmetic), and all three stop at the same step."""
+    flags = _baffled_hall()
+    nz, ny, nx = flags.shape
+    h = G.reference_cell_size()
+    assert abs(h - 343.0 / 88200.0) < 1e-9
+    row = _reference_row(h)
+    rcv = (4, 63, 70)
+    steps = 4500
+    print(f"reference default scheme, {nx}x{ny}x{nz}, {steps} steps requested", flush=True)
+    out = run_sequence(flags, h, row, steps, rcv)
+    shell = G.build_shell(flags)
+    np.testing.assert_array_equal(out["flags"], shell)
+    bidx = G.boundary_list(shell)
+    ridx = (rcv[2] * ny + rcv[1]) * nx + rcv[0]
+    co = np.asarray(row, np.float32).astype(np.float64)[None]
+    O.set_threads(os.cpu_count() or 1)
+    n_or = 4000
+    ir = O.Problem(shell, h, bidx=bidx, coeffs=co, rcv_idx=[ridx]).run(n_or, "f32")["ir"][0]
+    lit = O.Problem(shell, h, bidx=bidx, coeffs=co, rcv_idx=[ridx], state_f32=True).run(n_or, "f32")["ir"][0]
+    stop = O.rms_continue_sequence(ir.astype(np.float32))
+    stop_lit = O.rms_continue_sequence(lit.astype(np.float32))
+    print(f"oracle stop {stop}, reference-literal stop {stop_lit}, GPU ran {out['steps_run']}", flush=True)
+    assert 3300 < stop < n_or - 1, "the rule must fire well after step 3000 inside the oracle run"
+    assert stop_lit == stop
+    assert out["steps_run"] == stop + 1
+    np.testing.assert_array_equal(out["ir"], ir[: stop + 1].astype(np.float32))
+    rel = np.linalg.norm(out["ir"] - lit[: stop + 1]) / np.linalg.norm(lit[: stop + 1])
+    print(f"GPU vs reference-literal oracle: rel L2 {rel:.2e} over {stop + 1} steps", flush=True)
+    assert rel <= 1e-4
+    assert 0.0 < out["percent"]

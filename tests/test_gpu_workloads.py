@@ -1,0 +1,160 @@
+"""The benchmark workloads at full size on the GPU.
+
+Random room 512x384x256 (50.3 M cells, 6 seeded boxes + 4 tilted slabs,
+3 biquads, Gaussian-derivative source, 8 receivers): the FP32 path over the
+full 20000 steps against the committed impulse responses of the oracle's
+reference-literal run (float filter state, the arithmetic of
+simulator_gpu.cu:38-150; tests/golden/make_random_room_ir.py), and bit for
+bit against the oracle's own FP64-state FP32 run over the first 200 steps.
+
+Large room 1536x1024x768 (1.21 G cells, shoebox generator, 5 materials,
+8 receivers): FP16 storage against an FP64 GPU run of the same room, and the
+FP32 adjoint on a 512^3 corner crop against the FP64 adjoint."""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FIXTURE = os.path.join(ROOT, "tests", "golden", "random_512x384x256_ir.npz")
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def random_room():
+    from virs_amd import rooms
+    return rooms.prepare(rooms.random_room())
+
+
+def test_random_room_fp32_full_run_vs_reference_literal_oracle(gpu, random_room):
+    from virs_amd.fdtd import setup
+    z = np.load(FIXTURE)
+    steps = int(z["steps"])
+    _say(f"random room FP32, {steps} steps")
+    f = setup(random_room, "fp32", use_graph=True)
+    f.start(steps)
+    f.step(steps)
+    f.synchronize()
+    ir = f.receivers(True)
+    f.close()
+    ref = z["ir"].astype(np.float64)
+    assert ir.shape == ref.shape
+    rel = [rel_l2(ir[r], ref[r]) for r in range(len(ref))]
+    _say("per receiver rel L2 vs the reference-literal oracle: " + " ".join(f"{v:.2e}" for v in rel))
+    assert max(rel) <= 1e-4
+
+
+def test_random_room_fp32_first_steps_bit_exact(gpu, random_room):
+    from oracle import oracle as O
+    from virs_amd.fdtd import setup
+    steps = 200
+    room, sc = random_room.room, random_room.scenario
+    _say(f"random room FP32 vs the oracle, {steps} steps")
+    f = setup(random_room, "fp32", use_graph=True)
+    f.start(steps)
+    f.step(steps)
+    ir = f.receivers(True)
+    p, q = f.field(0), f.field(1)
+    f.close()
+    O.set_threads(os.cpu_count() or 1)
+    pb = O.Problem(room.flags, sc.h, dt=random_room.dt, bidx=room.bidx, bmat=room.bmat, coeffs=sc.coeffs,
+                   soft_idx=random_room.src_idx, soft_sig=random_room.src_sig, rcv_idx=random_room.rcv_idx)
+    ref = pb.run(steps, "f32", want_fields=True)
+    assert np.abs(ref["p"]).max() > 0
+    np.testing.assert_array_equal(ir, ref["ir"])
+    np.testing.assert_array_equal(p, ref["p"])
+    np.testing.assert_array_equal(q, ref["q"])
+
+
+# ---------------------------------------------------------------------------
+# 1536 x 1024 x 768
+
+LARGE_STEPS = 2000
+
+
+@pytest.fixture(scope="module")
+def large_room():
+    from virs_amd import rooms
+    _say("building the 1536x1024x768 room")
+    sc = rooms.shoebox_grid((1536, 1024, 768), nmat=5, steps=LARGE_STEPS, nrcv=8, seed=11)
+    return rooms.prepare(sc)
+
+
+def _edc_db(ir):
+    e = np.cumsum(ir[::-1].astype(np.float64) ** 2)[::-1]
+    return 10 * np.log10(np.maximum(e / e[0], 1e-300))
+
+
+def test_large_room_fp16_vs_fp64(gpu, large_room):
+    """FP16 storage (FP32 arithmetic) against the FP64 run of the same room
+    over LARGE_STEPS steps: finite, exponents inside their clamp, and the
+    relative L2 and energy-decay deviation at the receivers the pulse has
+    reached bounded at the measured level."""
+    from virs_amd.fdtd import setup
+    irs = {}
+    for prec in ("fp64", "fp16"):
+        _say(f"large room {prec}, {LARGE_STEPS} steps")
+        f = setup(large_room, prec, use_graph=True)
+        f.start(LARGE_STEPS)
+        f.step(LARGE_STEPS)
+        f.synchronize()
+        irs[prec] = f.receivers(True)
+        if prec == "fp16":
+            e = [f.field_exponent(w) for w in (0, 1)]
+            assert all(-60 <= v <= 60 for v in e)
+            _say(f"fp16 field exponents {e}")
+        f.close()
+    ref, got = irs["fp64"], irs["fp16"]
+    assert np.isfinite(got).all()
+    reached = [r for r in range(len(ref)) if np.abs(ref[r]).max() > 1e-6 * np.abs(ref).max()]
+    assert len(reached) >= 4, "the pulse must reach most receivers within the run"
+    rel = [rel_l2(got[r], ref[r]) for r in reached]
+    dev = 0.0
+    for r in reached:
+        a, b = _edc_db(got[r]), _edc_db(ref[r])
+        keep = b > -60.0
+        dev = max(dev, float(np.max(np.abs(a[keep] - b[keep]))))
+    _say(f"fp16 vs fp64 at {len(reached)} receivers: rel L2 max {max(rel):.2e} median "
+         f"{np.median(rel):.2e}, EDC deviation max {dev:.3f} dB")
+    assert max(rel) <= 2e-2
+    assert dev <= 1.0
+
+
+def test_large_room_crop_adjoint_fp32_vs_fp64(gpu, large_room):
+    """The FP32 adjoint on the 512^3 corner crop against the FP64 adjoint
+    (sqrt(N) checkpoints): relative gradient error <= 1e-4."""
+    from virs_amd import rooms
+    from virs_amd.fdtd import FDTD
+    steps, ncp = 800, 29
+    flags, bidx, bmat = rooms.crop(large_room, (512, 512, 512))
+    n = 512
+    cells = [((z * n + y) * n + x) for x, y, z in ((256, 256, 256), (200, 210, 190), (300, 190, 220),
+                                                   (180, 300, 260), (230, 250, 170))]
+    assert all(flags.reshape(-1)[c] == 0 for c in cells)
+    rng = np.random.default_rng(5)
+    dl = (rng.standard_normal((4, steps)) * 1e-3).astype(np.float32)
+    sc = large_room.scenario
+    grads = {}
+    for prec in ("fp64", "fp32"):
+        _say(f"512^3 crop adjoint {prec}, {steps} steps, {ncp} checkpoints")
+        g = FDTD((n, n, n), sc.h, dt=large_room.dt, precision=prec)
+        g.upload_flags(flags)
+        g.upload_boundary(bidx, bmat)
+        g.set_materials(sc.coeffs)
+        g.set_soft_sources(cells[:1], rooms.gauss_deriv(steps)[None])
+        g.set_receivers(cells[1:])
+        g.start(steps)
+        g.forward_record(steps, ncp)
+        grads[prec] = g.adjoint(dl)
+        g.close()
+    err = rel_l2(grads["fp32"], grads["fp64"])
+    _say(f"crop adjoint FP32 vs FP64 rel {err:.2e}, |grad| {np.linalg.norm(grads['fp64']):.3e}")
+    assert np.linalg.norm(grads["fp64"]) > 0
+    assert err <= 1e-4

@@ -236,6 +236,16 @@ __device__ inline void vstore(VT *p, const VT &v)
 
 __device__ inline float exp2i(int e) { return __int_as_float((e + 127) << 23); }
 
+// Orders a wave's LDS accesses around a lane-to-lane exchange inside the
+// wave: the hardware performs one wave's LDS operations in issue order, so
+// only the compiler's ordering needs fixing, which a wavefront-scope fence
+// does without an instruction.
+__device__ inline void wave_lds_fence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Divergence check of the list and point launches: a non-finite corrected
 // boundary value or receiver sample marks the run (StepCtl::err), which
 // virs_synchronize reports in every precision.
@@ -457,9 +467,12 @@ struct DenseParams {
     long long soft_len;
     Sponge sp;
     int sr;               // stochastic storage rounding (16-bit formats)
-    // fused boundary filter: entries of (chunk, z) are
-    // [bstart[z*nchunks_plane + chunk], bstart[... + 1]) of the compact list
-    const long long *bstart; // null: the boundary list runs in k_tail
+    // Boundary filter inside the stencil pass: the list entries whose cells
+    // lie in the 64*V cells of wave w of chunk ch at plane z are the
+    // contiguous slice wtab[(ch*kWaves + w)*nz + z] = {first entry, count}
+    // of the ascending list (rows z-contiguous, so a wave's consecutive
+    // planes share a scalar cache line).  Null: the list runs in k_tail.
+    const uint2 *wtab;
     const uint32_t *bidx;
     BoundaryState bs;
 };
@@ -560,8 +573,8 @@ __device__ inline C boundary_update(const BoundaryState &bs, long long e, long l
 // results equal a computation in physical units bit for bit.  Walls hold
 // zero and the code carries the centre weight kw (k_build_code16); SR
 // selects stochastic storage rounding.
-// FUSED: the boundary biquad of the block's list entries runs at the end of
-// the same launch.  ADJ: the reverse sweep applies the same operator to the adjoint
+// FUSED: the boundary biquad of the list entries runs inside the march (see
+// the boundary phase in the plane body).  ADJ: the reverse sweep applies the same operator to the adjoint
 // field; hard-source cells hold prescribed values, so their adjoint is zero.
 // HARD (16-bit storage): the room has flag-3 cells.  Without them a wall or
 // border cell (P = Q = 0 there) gets the update with a zero Laplacian
@@ -592,8 +605,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
         else if (n >= 0 && n < p.hard_len) hard = (C)p.hard_sig[n];
     }
 
-    // kq: Q -> P units; kout: P units -> R's stored units; kphys: P units -> physical
-    C kq = (C)1, kout = (C)1, kphys = (C)1;
+    // kq: Q -> P units; kout: P units -> R's stored units; kphys: P units -> physical;
+    // kst / kst_inv: physical -> R's stored units and back
+    C kq = (C)1, kout = (C)1, kphys = (C)1, kst = (C)1, kst_inv = (C)1;
     C hard_u = hard;
     if constexpr (S::kScaled) {
         const int ip = (int)((n - 1) & 3), iq = (int)((n - 2) & 3), ir = (int)(n & 3);
@@ -612,6 +626,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
         kq = exp2i(eP - eQ);
         kout = exp2i(eR - eP);
         kphys = exp2i(-eP);
+        kst = exp2i(eR);
+        kst_inv = exp2i(-eR);
         hard_u = hard * exp2i(eP);
         if (bid == 0) {
             if (threadIdx.x == 0) {
@@ -661,11 +677,34 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
 
     // cell codes of one plane: V bytes, or V nibbles in one word for 16-bit
     using CD = typename std::conditional<S::kScaled, uint32_t, VB>::type;
+    using Y = typename YsOf<T>::type;
     struct Stage {
         VT pp, q, yn, ys; // P(z+1), Q(z), P(z,y+1), P(z,y-1) as stored
         CD cd;            // cell codes of plane z
         T tl, tr;         // P(z) left of lane 0 / right of lane 63
+        // FUSED: the wave's list slice at plane z (first entry, count; wave
+        // uniform) and lane L's entry first + L of it: cell, material,
+        // filter-input and filter-output rings (lanes past the count hold
+        // the slice's last entry)
+        uint32_t efirst, ecount, eb, em;
+        C ex0, ex1;
+        Y ey0, ey1;
     };
+    // FUSED: the cells' (d, stored p^{n+1}) of one wave and plane, by
+    // position j in the lane's vector and lane, so an entry's lane can read
+    // and correct its cell's value (wave-private, no barrier)
+    struct alignas(2 * sizeof(C)) BSlot {
+        C d, r;
+    };
+    __shared__ BSlot bslot[FUSED ? kWaves : 1][FUSED ? V : 1][64];
+    // FUSED: the coefficient rows in LDS, so a lane's rows come without a
+    // global load issued behind the next plane's (whose wait would drain
+    // the prefetch)
+    __shared__ double cf_lds[FUSED ? 5 * 255 : 1];
+    if constexpr (FUSED) {
+        for (int k = threadIdx.x; k < 5 * p.bs.nmat; k += kBlock) cf_lds[k] = p.bs.coeffs[k];
+        __syncthreads();
+    }
 
     for (long long w = wbeg; w < wend;) {
         const int pchunk = (int)(w / nz);
@@ -701,6 +740,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
         uint32_t xoc = oc, xon = on, xos = os, xob = S::kScaled ? ob >> 1 : ob;
         // stochastic-rounding hash: cell0's part of the pair index times K1
         const uint32_t c0k1 = ((uint32_t)cell0 & ~1u) * kHashK1;
+        // FUSED: this wave's row of the slice table and its first cell in a plane
+        const uint2 *trow = nullptr;
+        uint32_t wcell0 = 0;
+        if constexpr (FUSED) {
+            const int wv = __builtin_amdgcn_readfirstlane(wave);
+            trow = p.wtab + ((long long)pchunk * kWaves + wv) * nz;
+            wcell0 = (uint32_t)(((long long)pchunk * kBlock + wv * 64) * V);
+        }
 
         auto issue = [&](int z, Stage &st) {
             const char *Pz = reinterpret_cast<const char *>(P + (long long)z * sxy);
@@ -722,6 +769,26 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
             st.ys = vload<VT, false>(reinterpret_cast<const VT *>(Pz + xos));
             if (need_l) st.tl = reinterpret_cast<const T *>(Pz + xc)[-1];
             if (need_r) st.tr = reinterpret_cast<const T *>(Pz + xc)[V];
+            if constexpr (FUSED) {
+                // every load goes out every plane, from an index clamped into
+                // the slice (one line when the slice is empty), so no load
+                // sits behind a branch
+                // a scalar load (constant address space): waiting for it
+                // counts scalar loads only, not the plane's vector loads
+                // issued just before
+                using ConstU64 = const __attribute__((address_space(4))) unsigned long long;
+                const unsigned long long tv = ((ConstU64 *)(const void *)trow)[z];
+                st.efirst = (uint32_t)tv;
+                st.ecount = (uint32_t)(tv >> 32);
+                const uint32_t k = min((uint32_t)lane, max(st.ecount, 1u) - 1u);
+                const long long ec = min((long long)st.efirst + k, p.bs.nb - 1);
+                st.eb = p.bidx[ec];
+                st.em = p.bs.bmat[ec];
+                st.ex0 = ((const C *)p.bs.xs[0])[ec];
+                st.ex1 = ((const C *)p.bs.xs[1])[ec];
+                st.ey0 = ((const Y *)p.bs.ys[0])[ec];
+                st.ey1 = ((const Y *)p.bs.ys[1])[ec];
+            }
         };
 
         C B0[V], B1[V], B2[V];
@@ -899,6 +966,60 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
 #pragma unroll
                 for (int j = 0; j < V; ++j) out.v[j] = S::st(o[j]);
             }
+            if constexpr (FUSED) {
+                // Boundary phase (simulator_gpu.cu:126-148): the list entries
+                // of this wave's cells at plane z.  Each cell publishes its
+                // filter input d = p^n - p^{n-1} (physical units) and its
+                // stored p^{n+1}; lane L takes entry first + L (64 per round),
+                // reads its cell's pair, runs the DF1 step and the correction
+                // exactly as the list launch does, stores the filter state
+                // (coalesced: consecutive entries on consecutive lanes) and
+                // writes the corrected stored value back for the cell's lane
+                // to store with the rest of its vector.
+                const uint32_t cnt = cur.ecount;
+                if (cnt != 0u) {
+#pragma unroll
+                    for (int j = 0; j < V; ++j) {
+                        const C d = S::kScaled ? PC[j] * kphys - qv[j] * kphys : PC[j] - qv[j];
+                        bslot[wave][j][lane] = BSlot{d, S::ld(out.v[j])};
+                    }
+                    wave_lds_fence();
+                    const uint32_t pbase = (uint32_t)((unsigned long long)z * (unsigned long long)sxy) + wcell0;
+                    const bool odd = (n & 1) != 0;
+                    for (uint32_t r0 = 0; r0 < cnt; r0 += 64) {
+                        uint32_t b = cur.eb, m = cur.em;
+                        C xa = cur.ex0, xb = cur.ex1;
+                        Y ya = cur.ey0, yb = cur.ey1;
+                        const long long e = (long long)cur.efirst + r0 + lane;
+                        if (r0 > 0) { // entries past the first 64 (floor and ceiling planes)
+                            const long long ec = min(e, (long long)cur.efirst + cnt - 1);
+                            b = p.bidx[ec];
+                            m = p.bs.bmat[ec];
+                            xa = ((const C *)p.bs.xs[0])[ec];
+                            xb = ((const C *)p.bs.xs[1])[ec];
+                            ya = ((const Y *)p.bs.ys[0])[ec];
+                            yb = ((const Y *)p.bs.ys[1])[ec];
+                        }
+                        if ((uint32_t)lane < cnt - r0) {
+                            const uint32_t off = b - pbase;
+                            BSlot &sl = bslot[wave][off % (uint32_t)V][off / (uint32_t)V];
+                            const C dif = sl.d;
+                            const double in = (double)dif;
+                            const double x1 = (double)(odd ? xa : xb), x2 = (double)(odd ? xb : xa);
+                            const double y1 = (double)(odd ? ya : yb), y2 = (double)(odd ? yb : ya);
+                            const double yv = biquad_out(cf_lds + 5 * m, in, x1, x2, y1, y2);
+                            const C r = corrected<C>(sl.r * kst_inv, p.bs.kconst, yv, in);
+                            flag_nonfinite(p.ctl, r);
+                            const T t = store_round<T>(r * kst, SR, (unsigned long long)b, n, kSaltBoundary);
+                            store_taps<C, Y>(p.bs, e, n, dif, yv);
+                            sl.r = S::ld(t);
+                        }
+                    }
+                    wave_lds_fence();
+#pragma unroll
+                    for (int j = 0; j < V; ++j) out.v[j] = S::st(bslot[wave][j][lane].r);
+                }
+            }
             if (active) vstore<VT, true>(reinterpret_cast<VT *>(reinterpret_cast<char *>(R + (long long)z * sxy) + xoc), out);
         };
 
@@ -915,98 +1036,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
             body(z + 4, B1, B2, B0, SA, SB);
             if (z + 5 >= z1) break;
             body(z + 5, B2, B0, B1, SB, SA);
-        }
-    }
-
-    // FUSED: the boundary biquad for the list entries inside this block's
-    // (chunk, z) planes.  Each such cell was written above by this block and
-    // its correction reads only p^n, p^{n-1} and that value, so a block-level
-    // barrier orders it -- no cross-block dependency.  All the block's
-    // entries are handled as one batch (virtual index over its planes'
-    // list slices), and each is stored exactly as the separate list launch
-    // would store it.
-    if constexpr (FUSED) {
-        __shared__ long long pstart[kBlock + 1];
-        __shared__ long long pbase[kBlock];
-        __shared__ long long wtot[kWaves];
-        const long long np = wend - wbeg;
-        __syncthreads();
-        for (long long k0 = 0; k0 < np; k0 += kBlock) {
-            const int cnt = (int)min((long long)kBlock, np - k0);
-            // thread k: list slice of plane k of the batch; block-wide
-            // exclusive scan of the slice lengths
-            long long len = 0, base0 = 0;
-            if ((int)threadIdx.x < cnt) {
-                const long long w = wbeg + k0 + threadIdx.x;
-                const long long ch = w / nz, zz = w - ch * nz;
-                const long long t = zz * p.nchunks_plane + ch;
-                base0 = p.bstart[t];
-                len = p.bstart[t + 1] - base0;
-            }
-            long long inc = len;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const long long up = __shfl_up(inc, o);
-                if (lane >= o) inc += up;
-            }
-            if (lane == 63) wtot[wave] = inc;
-            __syncthreads();
-            long long off = 0;
-            for (int k = 0; k < wave; ++k) off += wtot[k];
-            pstart[threadIdx.x] = off + inc - len;
-            pbase[threadIdx.x] = base0;
-            if (threadIdx.x == kBlock - 1) pstart[kBlock] = off + inc;
-            __syncthreads();
-            const long long total = pstart[kBlock];
-            // entries in batches of U per thread: all loads of a batch are
-            // issued before any of its stores
-            constexpr int U = 1; // more entries per batch raise the whole kernel's VGPR count
-            for (long long v0 = threadIdx.x; v0 < total; v0 += (long long)kBlock * U) {
-                long long ev[U], bv[U];
-                bool ok[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const long long vi = v0 + (long long)u * kBlock;
-                    ok[u] = vi < total;
-                    int lo = 0, hi = cnt; // largest k with pstart[k] <= vi
-                    while (hi - lo > 1) {
-                        const int mid = (lo + hi) >> 1;
-                        if (pstart[mid] <= vi) lo = mid;
-                        else hi = mid;
-                    }
-                    ev[u] = ok[u] ? pbase[lo] + (vi - pstart[lo]) : 0;
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) bv[u] = ok[u] ? (long long)p.bidx[ev[u]] : 0;
-                C pv[U], qv2[U], rv[U];
-                double x1[U], x2[U], y1[U], y2[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    pv[u] = S::ld(P[bv[u]]);
-                    qv2[u] = S::ld(Q[bv[u]]);
-                    rv[u] = S::ld(R[bv[u]]);
-                    load_taps<C, typename YsOf<T>::type>(p.bs, ev[u], n, x1[u], x2[u], y1[u], y2[u]);
-                }
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    if (!ok[u]) continue;
-                    C pp = pv[u], qq = qv2[u], r = rv[u];
-                    if constexpr (S::kScaled) {
-                        // physical units for the filter and the correction
-                        pp = pp * kphys;
-                        qq = qq * kphys * kq;
-                        r = r * kphys / kout;
-                    }
-                    r = boundary_update_pre<C, typename YsOf<T>::type>(p.bs, ev[u], n, pp - qq, r, x1[u], x2[u], y1[u], y2[u]);
-                    if constexpr (S::kScaled) {
-                        R[bv[u]] = store_round<T>(r * (kout / kphys), SR, (unsigned long long)bv[u], n,
-                                                  kSaltBoundary);
-                    } else {
-                        R[bv[u]] = S::st(r);
-                    }
-                }
-            }
-            __syncthreads();
         }
     }
 

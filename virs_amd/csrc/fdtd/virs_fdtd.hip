@@ -40,7 +40,9 @@ struct virs_ctx {
     long long n_hard = 0;          // interior flag-3 cells (hard source)
     std::vector<uint32_t> h_flag2; // interior flag-2 cells of the uploaded flags
     int nchunks_plane = 0;         // 256-thread chunks per z-plane of the stencil
-    long long *bstart = nullptr;   // [nz*nchunks_plane + 1] list offsets per (chunk, z)
+    // [nchunks_plane][4 waves][nz] {first entry, count}: the list slice of
+    // each wave's cells per plane (DenseParams::wtab)
+    uint2 *wtab = nullptr;
     bool fused = false;            // boundary filter runs inside the stencil pass
     long long *neg = nullptr;      // kMaxPoints x -1: point positions when fused
     int cus = 0;
@@ -283,7 +285,7 @@ DenseParams dense_params(const virs_ctx *c, const void *P, const void *Q, void *
     p.soft_len = c->soft_len;
     p.sp = c->sp;
     p.sr = c->d.rounding == VIRS_ROUND_STOCHASTIC;
-    p.bstart = nullptr;
+    p.wtab = nullptr;
     p.bs = boundary_state(c, c->st);
     return p;
 }
@@ -385,13 +387,13 @@ template <typename T, int V, bool HARD>
 hipError_t launch_dense_s16(const virs_ctx *c, const DenseParams &p, bool sponge)
 {
     if (p.sr) {
-        if (p.bstart)
+        if (p.wtab)
             return sponge ? launch_dense_k<T, V, false, true, true, true, HARD>(c, p)
                           : launch_dense_k<T, V, false, true, false, true, HARD>(c, p);
         return sponge ? launch_dense_k<T, V, false, false, true, true, HARD>(c, p)
                       : launch_dense_k<T, V, false, false, false, true, HARD>(c, p);
     }
-    if (p.bstart)
+    if (p.wtab)
         return sponge ? launch_dense_k<T, V, false, true, true, false, HARD>(c, p)
                       : launch_dense_k<T, V, false, true, false, false, HARD>(c, p);
     return sponge ? launch_dense_k<T, V, false, false, true, false, HARD>(c, p)
@@ -412,7 +414,7 @@ hipError_t launch_dense_v(const virs_ctx *c, const DenseParams &p)
             return launch_dense_s16<T, V, false>(c, p, sponge);
         } else {
             if constexpr (!ADJ) {
-                if (p.bstart) {
+                if (p.wtab) {
                     return sponge ? launch_dense_k<T, V, false, true, true, false>(c, p)
                                   : launch_dense_k<T, V, false, true, false, false>(c, p);
                 }
@@ -517,7 +519,7 @@ int launch_step(virs_ctx *c, void *trace_d = nullptr, void *trace_y = nullptr,
     bs.trace_base = trace_base;
     bs.trace_slots = c->trace_slots;
     if (c->fused) {
-        p.bstart = c->bstart;
+        p.wtab = c->wtab;
         p.bidx = c->bidx;
         t.bs.nb = 0;
         t.soft_bpos = c->neg;
@@ -539,24 +541,37 @@ int launch_step(virs_ctx *c, void *trace_d = nullptr, void *trace_y = nullptr,
     return VIRS_OK;
 }
 
-// Builds the per-(chunk, z) offsets into the ascending boundary list that
-// let the stencil launch run the boundary filter of its own planes.
+// Builds the slice table that lets the stencil launch run the boundary
+// filter of its own cells: for wave w of chunk ch at plane z, the list
+// entries whose cells lie in the wave's 64*V cells of the plane
+// ([(ch*256 + w*64)*V, +64*V) clipped to the plane) are the contiguous
+// slice {first, count} of the ascending list.  One walk over the list.
 int refresh_fusion(virs_ctx *c)
 {
-    dfree(c->bstart);
+    dfree(c->wtab);
     c->fused = false;
     if (!c->d.fuse_boundary || c->nb == 0) return VIRS_OK;
-    const long long nch = c->nchunks_plane, chunk_cells = (long long)kBlock * c->vw;
-    const long long sxy = c->nx * c->ny;
-    std::vector<long long> cnt((size_t)(c->nz * nch + 1), 0);
-    for (uint32_t idx : c->h_bidx) {
-        const long long z = idx / sxy, r = idx - z * sxy;
-        ++cnt[(size_t)(z * nch + r / chunk_cells) + 1];
-    }
-    for (size_t k = 1; k < cnt.size(); ++k) cnt[k] += cnt[k - 1];
-    int rc = dalloc(c, &c->bstart, sizeof(long long) * cnt.size());
+    const long long nch = c->nchunks_plane, sxy = c->nx * c->ny, nz = c->nz;
+    const long long wcells = 64LL * c->vw;
+    std::vector<uint32_t> tab((size_t)(nch * kWaves * nz) * 2);
+    size_t e = 0;
+    const size_t nb = c->h_bidx.size();
+    for (long long z = 0; z < nz; ++z)
+        for (long long ch = 0; ch < nch; ++ch)
+            for (long long w = 0; w < kWaves; ++w) {
+                const long long lo = std::min(sxy, (ch * kWaves + w) * wcells), hi = std::min(sxy, lo + wcells);
+                const long long glo = z * sxy + lo, ghi = z * sxy + hi;
+                while (e < nb && (long long)c->h_bidx[e] < glo) ++e;
+                size_t f = e;
+                while (f < nb && (long long)c->h_bidx[f] < ghi) ++f;
+                const size_t row = (size_t)((ch * kWaves + w) * nz + z);
+                tab[2 * row] = (uint32_t)e;
+                tab[2 * row + 1] = (uint32_t)(f - e);
+                e = f;
+            }
+    int rc = dalloc(c, &c->wtab, sizeof(uint32_t) * tab.size());
     if (rc) return rc;
-    HIPC(c, hipMemcpy(c->bstart, cnt.data(), sizeof(long long) * cnt.size(), hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(c->wtab, tab.data(), sizeof(uint32_t) * tab.size(), hipMemcpyHostToDevice));
     c->fused = true;
     return VIRS_OK;
 }
@@ -774,7 +789,7 @@ int virs_destroy(virs_ctx *c)
     dfree(c->rcv_bpos);
     dfree(c->rcv_aux);
     dfree(c->neg);
-    dfree(c->bstart);
+    dfree(c->wtab);
     dfree(c->ir_buf);
     for (int k = 0; k < 2; ++k) {
         dfree(c->trace_d[k]);
@@ -1465,6 +1480,7 @@ int virs_memory_bytes(const virs_ctx *c, int64_t *bytes)
     for (auto p : c->af)
         if (p) b += (long long)c->esz * c->N;
     if (c->snap_host) b += (long long)kSnapSlots * c->nx * c->nz * (long long)sizeof(float);
+    if (c->wtab) b += (long long)sizeof(uint2) * c->nchunks_plane * kWaves * c->nz;
     *bytes = b;
     return VIRS_OK;
 }
