@@ -162,6 +162,17 @@ int virs_read_field_f64(virs_ctx *ctx, int32_t which, double *out);
  * fails with VIRS_ERR_ARG. */
 int virs_plane_request(virs_ctx *ctx, int64_t layer, int32_t *ticket);
 int virs_plane_fetch(virs_ctx *ctx, int32_t ticket, float *out);
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the snapshot of virs_plane_request + virs_plane_fetch, taken
+ * synchronously behind the steps already queued, with no allocation at all.
+ * The plane is gathered into a fixed device staging array of 2^21 floats
+ * (8 MiB) that each GPU the library runs on holds until the process ends,
+ * then copied into out[z*Nx + x] (Nz*Nx floats, the caller's buffer).
+ * VIRS_ERR_ARG for a layer outside [0, Ny) or a plane above 2^21 cells.
+ * Every call on one GPU shares that GPU's staging array: calls that overlap
+ * on one GPU (other threads, other contexts on the same device) race. */
+int virs_plane_read_noalloc(virs_ctx *ctx, int64_t layer, float *out);
 /* updateGPUFromGrid, field part: set p^n (which 0) or p^{n-1} (which 1). */
 int virs_write_field_f64(virs_ctx *ctx, int32_t which, const double *in);
 /* Power-of-two exponent e of a stored field (which as for virs_read_field;

@@ -98,6 +98,7 @@ def hip():
     _sig(L, "virs_adjoint", C.c_int, [vp, c_fp, c_dp])
     _sig(L, "virs_plane_request", C.c_int, [vp, C.c_int64, c_i32p])
     _sig(L, "virs_plane_fetch", C.c_int, [vp, C.c_int32, c_fp])
+    _sig(L, "virs_plane_read_noalloc", C.c_int, [vp, C.c_int64, c_fp])
     _hip = L
     return L
 
@@ -143,7 +144,7 @@ HIP_SYMBOLS = [
     "virs_percent_to_target_rms", "virs_read_receivers", "virs_read_receivers_f64",
     "virs_read_field", "virs_read_field_f64", "virs_write_field_f64", "virs_field_exponent",
     "virs_memory_bytes", "virs_time_steps", "virs_forward_record", "virs_adjoint",
-    "virs_plane_request", "virs_plane_fetch",
+    "virs_plane_request", "virs_plane_fetch", "virs_plane_read_noalloc",
 ]
 HOST_SYMBOLS = [
     "virs_mesh_load_obj", "virs_mesh_from_arrays", "virs_mesh_num_triangles", "virs_mesh_copy",

@@ -16,12 +16,23 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace virs;
 
 constexpr int kSnapSlots = 4; // layer snapshots outstanding at once
+
+// Device staging of virs_plane_read_noalloc: one XZ plane of at most
+// kStaticPlaneCells floats (8 MiB), held by every GPU the library runs on
+// until the process ends.  Host code never passes the array's name to a
+// kernel (on the host it is a stand-in, not a device address): virs_create
+// resolves each device's copy once with hipGetSymbolAddress, with that device
+// current, and the entry point uses that address.
+constexpr long long kStaticPlaneCells = 2LL * 1024 * 1024;
+static __device__ __align__(256) float scratch[kStaticPlaneCells];
+constexpr int kMaxDevices = 64;
 
 struct virs_ctx {
     virs_desc d{};
@@ -136,6 +147,8 @@ struct virs_ctx {
     int snap_ticket[kSnapSlots] = {-1, -1, -1, -1};
     int snap_next = 0;
 
+    float *scratch_dev = nullptr; // this device's address of `scratch`
+
     std::string err;
 };
 
@@ -179,6 +192,26 @@ void dfree(P *&p)
 {
     if (p) (void)hipFree((void *)p);
     p = nullptr;
+}
+
+// The device address of `scratch` on the current device (ordinal dev),
+// resolved once per device and kept.
+int static_scratch(virs_ctx *c, int dev, float **out)
+{
+    static std::mutex mu;
+    static float *addr[kMaxDevices] = {};
+    if (dev < 0 || dev >= kMaxDevices) return fail(c, VIRS_ERR_ARG, "device %d beyond %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(c, VIRS_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                        hipGetErrorString(e));
+        addr[dev] = (float *)p;
+    }
+    *out = addr[dev];
+    return VIRS_OK;
 }
 
 int grid_for(long long n) { return (int)std::min<long long>(std::max<long long>((n + 255) / 256, 1), 65536); }
@@ -729,8 +762,8 @@ int virs_create(const virs_desc *d, virs_ctx **out)
         delete c;
         return VIRS_ERR_HIP;
     }
-    int rc = VIRS_OK;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = VIRS_ERR_HIP;
+    int rc = static_scratch(c, d->device, &c->scratch_dev);
+    if (!rc && hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = VIRS_ERR_HIP;
     for (int k = 0; k < 3 && !rc; ++k) rc = dalloc(c, &c->f[k], c->esz * (size_t)c->N);
     if (!rc) rc = dalloc(c, &c->code, code_bytes(c));
     if (!rc) rc = dalloc(c, &c->ctl, sizeof(StepCtl));
@@ -1464,6 +1497,45 @@ int virs_plane_fetch(virs_ctx *c, int32_t ticket, float *out)
     const size_t plane = (size_t)c->nx * c->nz;
     std::memcpy(out, c->snap_host + slot * plane, plane * sizeof(float));
     c->snap_ticket[slot] = -1;
+    return VIRS_OK;
+}
+
+int virs_plane_read_noalloc(virs_ctx *c, int64_t layer, float *out)
+{
+    if (!c || !out) return VIRS_ERR_ARG;
+    if (layer < 0 || layer >= c->ny)
+        return fail(c, VIRS_ERR_ARG, "layer %lld out of range (Ny = %lld)", (long long)layer, c->ny);
+    if (c->nx * c->nz > kStaticPlaneCells)
+        return fail(c, VIRS_ERR_ARG, "plane of %lld cells exceeds the static staging of %lld cells",
+                    c->nx * c->nz, kStaticPlaneCells);
+    if (int rc = check_not_mid_adjoint(c)) return rc;
+    HIPC(c, hipSetDevice(c->d.device));
+    float *dst = c->scratch_dev;
+    const void *src = c->f[c->ip];
+    const dim3 grid((unsigned)((c->nx + 255) / 256), (unsigned)std::min<long long>(c->nz, 65535));
+    const int nx = (int)c->nx, ny = (int)c->ny, nz = (int)c->nz, y = (int)layer;
+    switch (c->d.precision) {
+    case VIRS_FP16:
+        hipLaunchKernelGGL(k_plane_y<f16>, grid, dim3(256), 0, c->stream, (const f16 *)src, c->ctl, c->ip, dst, nx,
+                           ny, nz, y);
+        break;
+    case VIRS_BF16:
+        hipLaunchKernelGGL(k_plane_y<bf16>, grid, dim3(256), 0, c->stream, (const bf16 *)src, c->ctl, c->ip, dst,
+                           nx, ny, nz, y);
+        break;
+    case VIRS_FP64:
+        hipLaunchKernelGGL(k_plane_y<double>, grid, dim3(256), 0, c->stream, (const double *)src, c->ctl, c->ip,
+                           dst, nx, ny, nz, y);
+        break;
+    default:
+        hipLaunchKernelGGL(k_plane_y<float>, grid, dim3(256), 0, c->stream, (const float *)src, c->ctl, c->ip, dst,
+                           nx, ny, nz, y);
+        break;
+    }
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipMemcpyAsync(out, dst, sizeof(float) * (size_t)c->nx * (size_t)c->nz, hipMemcpyDeviceToHost,
+                           c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
     return VIRS_OK;
 }
 
