@@ -222,3 +222,76 @@ def test_reference_default_scheme_with_rms_stop(gpu):
     print(f"GPU vs reference-literal oracle: rel L2 {rel:.2e} over {stop + 1} steps", flush=True)
     assert rel <= 1e-4
     assert 0.0 < out["percent"]
+
+
+@pytest.mark.gpu
+def test_shipped_room_at_reference_defaults(gpu):
+    """The reference's default room (assets/room_with_source2.obj as
+    tests/golden/room_with_source2.npz holds it) at its defaults through the
+    dispatch adapter: h = 343/88200 on the 1337 x 900 x 1079 grid (1.30 G
+    cells, 15.1 M list entries), the RBJ row, the flag-3 pulse of the source
+    mesh and the fixed receiver (334, 225, 270), 2000 steps (the pulse
+    reaches the receiver after ~1600).  The shell the adapter builds has the
+    recorded digest, the IR is finite and reaches the receiver, and two runs
+    are bit-identical.  A 128^3 crop around the source then runs bit-exact
+    against the oracle."""
+    import hashlib
+    z = np.load(os.path.join(ROOT, "tests", "golden", "room_with_source2.npz"))
+    mesh = G.mesh_from_arrays(z["tri"], z["mat"])
+    h = G.reference_cell_size()
+    dims = tuple(int(v) for v in z["dims"])
+    pre = G.voxelize(mesh, z["origin"], h, dims)[0]
+    row = _reference_row(h)
+    steps = 2000
+    runs = []
+    for k in range(2):
+        print(f"shipped room run {k}: {dims}, {steps} steps", flush=True)
+        out = run_sequence(pre, h, row, steps, (334, 225, 270))
+        assert out["steps_run"] == steps and out["nb"] == int(z["nb"])
+        if k == 0:
+            assert hashlib.sha256(out["flags"].tobytes()).hexdigest() == str(z["flags_sha256"])
+        runs.append((out["ir"].copy(), out["p"]))
+        del out
+    ir, p = runs[0]
+    assert np.isfinite(ir).all() and np.abs(ir).max() > 0
+    np.testing.assert_array_equal(runs[1][0], ir)
+    np.testing.assert_array_equal(runs[1][1], p)
+    del runs, p
+    # two 128^3 crops bit-exact against the oracle: around the source mesh
+    # (the flag-3 pulse, no walls nearby) and around the receiver (walls and
+    # boundary filters, driven by a soft Gaussian-derivative source)
+    from virs_amd import rooms
+    from virs_amd.fdtd import FDTD
+    shell = G.build_shell(pre)
+    src = np.argwhere(pre == 3)
+    row64 = np.asarray(row, np.float32).astype(np.float64)[None]
+    O.set_threads(os.cpu_count() or 1)
+    n_crop = 300
+    for name, lo in (("source", (src.min(0) + src.max(0)) // 2 - 64), ("receiver", np.array([270, 225, 334]) - 64)):
+        crop = np.ascontiguousarray(shell[lo[0]:lo[0] + 128, lo[1]:lo[1] + 128, lo[2]:lo[2] + 128])
+        bidx = G.boundary_list(crop)
+        air = np.flatnonzero(crop.reshape(-1) == 0)
+        rcv = air[[len(air) // 5, len(air) // 2, (4 * len(air)) // 5]]
+        soft = [] if name == "source" else [int(air[(2 * len(air)) // 5])]
+        sig = rooms.gauss_deriv(n_crop)[None] if soft else None
+        if name == "source":
+            assert (crop == 3).any()
+        else:
+            assert (crop == 2).any() and not (crop == 3).any()
+        print(f"{name} crop at {lo.tolist()}: nb {len(bidx)}, {n_crop} steps", flush=True)
+        f = FDTD((128, 128, 128), h)
+        f.upload_flags(crop)
+        f.upload_boundary(bidx)
+        f.set_materials(row64)
+        if soft:
+            f.set_soft_sources(soft, sig)
+        f.set_receivers(rcv)
+        f.start(n_crop)
+        f.step(n_crop)
+        gir, gp = f.receivers(True), f.field(0)
+        f.close()
+        ref = O.Problem(crop, h, bidx=bidx, coeffs=row64, rcv_idx=rcv, soft_idx=soft,
+                        soft_sig=sig).run(n_crop, "f32", want_fields=True)
+        assert np.abs(ref["p"]).max() > 0
+        np.testing.assert_array_equal(gir, ref["ir"])
+        np.testing.assert_array_equal(gp, ref["p"])

@@ -32,3 +32,32 @@ def test_random_room_fixture_describes_this_room():
     ir = z["ir"]
     assert ir.dtype == np.float32 and ir.shape == (len(prep.rcv_idx), int(z["steps"]))
     assert np.isfinite(ir).all() and (np.abs(ir).max(axis=1) > 0).all()
+
+
+REF_ROOM = os.path.join(ROOT, "tests", "golden", "room_with_source2.npz")
+
+
+def test_reference_room_fixture_rebuilds_at_reference_defaults():
+    """The shipped room (tests/golden/room_with_source2.npz, written by
+    make_reference_room.py from assets/room_with_source2.obj) built by the
+    host pipeline at the reference's cell size and bounds rule: the grid
+    1337 x 900 x 1079, the list length and the digests of flags, normals,
+    list and materials the fixture recorded; and the count of voxels that
+    the other quad diagonal changes (the exposure to tinyobjloader's
+    unpinned quad rule, simulator.cpp:279)."""
+    from virs_amd import grid as G
+    z = np.load(REF_ROOM)
+    mesh = G.mesh_from_arrays(z["tri"], z["mat"])
+    room = G.build_room(mesh)
+    assert tuple(room.dims) == (1337, 900, 1079) == tuple(z["dims"])
+    assert np.float32(room.h) == z["h"]
+    np.testing.assert_array_equal(room.origin, z["origin"])
+    assert len(room.bidx) == int(z["nb"])
+    np.testing.assert_array_equal(np.bincount(room.flags.reshape(-1), minlength=4), z["counts"])
+    assert _digest(room.flags) == str(z["flags_sha256"])
+    assert _digest(room.normals) == str(z["normals_sha256"])
+    assert _digest(room.bidx) == str(z["bidx_sha256"])
+    assert _digest(room.bmat) == str(z["bmat_sha256"])
+    pre = G.voxelize(mesh, room.origin, room.h, room.dims)[0]
+    alt = G.voxelize(G.mesh_from_arrays(z["alt_tri"], z["mat"]), room.origin, room.h, room.dims)[0]
+    assert int(np.count_nonzero(pre != alt)) == int(z["diag_changed_pre_shell"])
