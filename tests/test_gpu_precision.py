@@ -94,6 +94,13 @@ def test_fp32_vs_fp64(shoebox_runs):
     assert max(errs) <= 1e-4
 
 
+# The goal for FP16 storage is <= 1e-3 relative IR error against FP64.  It is
+# not met: each 16-bit store rounds with a relative error of ~2^-11/sqrt(12),
+# which the undamped leapfrog accumulates like a random walk, and the measured
+# levels on this 2000-step shoebox are 4.5e-3 (FP16 stochastic), 5.0e-3 (FP16
+# nearest) and 3.4e-2 (BF16).  The bounds below sit just above those
+# measured levels, so a regression in the 16-bit path fails them; they do not
+# restate the goal.
 @pytest.mark.parametrize("key,bound", [("fp16", 6e-3), ("fp16_rne", 1e-2), ("bf16", 5e-2)])
 def test_16bit_error_vs_fp64(shoebox_runs, key, bound):
     ref = shoebox_runs["fp64"]
