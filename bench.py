@@ -19,8 +19,8 @@ step_hbm_frac: the whole step's algorithmic bytes (stencil, boundary
 filter, receivers) per step time; step_hbm_frac_fp64_state_formula prices
 the same step with N(3s+1) + Nb(69+4s) + R s instead.
 cpu_baseline: the reference's own CPU step (simulator_cpu.cpp:55-81,
-restated in C with OpenMP over z) on a 128^3 corner crop of the same room,
-on this host's cores; the CPU oracle of the GPU scheme is reported beside it.
+restated in C with OpenMP over z) on the same room for about 10 s, on this
+host's cores; the CPU oracle of the GPU scheme is reported beside it.
 error_vs_fp64: the 512x384x256 room run for its full 20000 steps in FP32,
 FP16 and BF16 storage against its FP64 run (8 receivers).
 """
@@ -136,35 +136,40 @@ def launch_label(tm):
     return " + ".join(parts)
 
 
-def cpu_baseline(prep, crop=128, steps=200, warmup=5):
-    from virs_amd import rooms
+def cpu_baseline(prep, ref_seconds=10.0, oracle_seconds=5.0, warmup=5):
+    """The reference's own CPU step (simulator_cpu.cpp:55-81, restated in C
+    with OpenMP over z) on the whole 512x384x256 room, on this host's cores:
+    a short calibration run sizes a run of about ref_seconds, which is timed.
+    The GPU scheme restated on the CPU (the oracle, same threads) is timed
+    the same way for about oracle_seconds and reported beside it."""
     from oracle import oracle as O
-    flags, bidx, bmat = rooms.crop(prep, (crop, crop, crop))
+    room, sc = prep.room, prep.scenario
+    n = room.n
     cores = host_cores()
     O.set_threads(cores)
-    sc = prep.scenario
-    n = crop ** 3
-    centre = (crop // 2 * crop + crop // 2) * crop + crop // 2
-    # the reference's own CPU step (simulator_cpu.cpp:55-81, a damped scheme
-    # without boundary filters) on the crop: warm-up, then the timed steps
-    O.refcpu_steps(flags, sc.h, warmup)
-    t1 = time.perf_counter()
-    O.refcpu_steps(flags, sc.h, steps)
-    tr = time.perf_counter() - t1
-    # the GPU scheme restated on the CPU (the oracle), same crop
-    pb = O.Problem(flags, sc.h, dt=prep.dt, bidx=bidx, bmat=bmat, coeffs=sc.coeffs,
-                   soft_idx=[centre], soft_sig=prep.src_sig[:1, : warmup + steps], rcv_idx=[centre])
-    pb.run(warmup, "f32")
+    O.refcpu_steps(room.flags, sc.h, warmup)
     t0 = time.perf_counter()
-    pb.run(steps, "f32")
-    t = time.perf_counter() - t0
+    O.refcpu_steps(room.flags, sc.h, 10)
+    steps = max(20, int(ref_seconds / ((time.perf_counter() - t0) / 10)))
+    t0 = time.perf_counter()
+    O.refcpu_steps(room.flags, sc.h, steps)
+    t_ref = time.perf_counter() - t0
+    pb = O.Problem(room.flags, sc.h, dt=prep.dt, bidx=room.bidx, bmat=room.bmat, coeffs=sc.coeffs,
+                   soft_idx=prep.src_idx, soft_sig=prep.src_sig, rcv_idx=prep.rcv_idx)
+    t0 = time.perf_counter()
+    pb.run(5, "f32")
+    osteps = max(10, int(oracle_seconds / ((time.perf_counter() - t0) / 5)))
+    t0 = time.perf_counter()
+    pb.run(osteps, "f32")
+    t_or = time.perf_counter() - t0
     return {
-        "value": round(n * steps / tr / 1e9, 4), "unit": "Gvoxel-updates/s", "cores": cores,
+        "value": round(n * steps / t_ref / 1e9, 4), "unit": "Gvoxel-updates/s", "cores": cores,
         "kind": "port",
-        "sample": f"{crop}^3 corner crop of the 512x384x256 room, {steps} steps after {warmup} "
-                  f"warm-up: the reference's CPU step (simulator_cpu.cpp:55-81) restated in C, "
-                  f"OpenMP over z",
-        "oracle_same_scheme_gvox_s": round(n * steps / t / 1e9, 4),
+        "sample": f"the whole 512x384x256 room, {steps} steps ({t_ref:.1f} s) after {warmup} warm-up: the "
+                  f"reference's CPU step (simulator_cpu.cpp:55-81) restated in C, OpenMP over z",
+        "oracle_same_scheme_gvox_s": round(n * osteps / t_or / 1e9, 4),
+        "oracle_sample": f"the same room, {osteps} steps ({t_or:.1f} s) of the FP32 oracle (the GPU scheme: "
+                         f"boundary filters, source, receivers)",
     }
 
 
