@@ -6,12 +6,15 @@
 - dL/dIR must have the recorded shape;
 - a diverging run is reported by virs_synchronize in every precision;
 - virs_time_steps times exactly the launches virs_step makes (graphs of any
-  length), after an untimed sampling pass."""
+  length), after an untimed sampling pass;
+- checkpoints beyond the device memory fail with an error code, and the
+  context stays usable."""
 import numpy as np
 import pytest
 
 from helpers import gpu_context, tiny_problem
-from virs_amd.fdtd import FDTDError
+from virs_amd import grid as G
+from virs_amd.fdtd import FDTD, FDTDError
 
 pytestmark = pytest.mark.gpu
 
@@ -149,3 +152,32 @@ def test_timed_steps_are_virs_steps(gpu, n):
     np.testing.assert_array_equal(u.receivers(True), b.receivers(True))
     for f in (a, b, c, u):
         f.close()
+
+
+def test_checkpoints_beyond_device_memory_fail_with_an_error(gpu):
+    """SURVEY's allocation-failure case forced from the host: 4000 FP64
+    checkpoints of a 256^3 room (2 x 134 MB each, ~1.1 TB) fail at once with
+    VIRS_ERR_NOMEM, without touching the device's memory, and the context
+    records and differentiates normally afterwards."""
+    n = 256
+    flags = np.zeros((n, n, n), np.uint8)
+    flags[1:-1, 1:-1, 1] = 1
+    flags = G.build_shell(flags)
+    bidx = G.boundary_list(flags)
+    f = FDTD((n, n, n), 0.05, lam2=1 / 3, precision="fp64")
+    f.upload_flags(flags)
+    f.upload_boundary(bidx)
+    f.set_materials([[0.6, -0.2, 0.05, -0.3, 0.1]])
+    c = (n // 2 * n + n // 2) * n + n // 2
+    f.set_soft_sources([c], np.ones((1, 4000)))
+    f.set_receivers([c + 7])
+    f.start(4000)
+    with pytest.raises(FDTDError, match="checkpoints need"):
+        f.forward_record(4000, 4000)
+    with pytest.raises(FDTDError, match="forward_record first"):
+        f.adjoint(np.zeros((1, 4000), np.float32))
+    f.start(40)
+    f.forward_record(40, 4)
+    g = f.adjoint(np.ones((1, 40), np.float32))
+    assert np.isfinite(g).all()
+    f.close()

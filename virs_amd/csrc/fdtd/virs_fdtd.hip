@@ -1743,12 +1743,24 @@ int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
         ncp = whole <= 0.25 * (double)free_b ? 1 : (int)std::ceil(std::sqrt((double)nsteps));
     }
     ncp = (int)std::min<long long>(ncp, nsteps);
-    c->seg = (nsteps + ncp - 1) / ncp;
-    ncp = (int)((nsteps + c->seg - 1) / c->seg);
-    // (re)allocate snapshots
+    const long long seg = (nsteps + ncp - 1) / ncp;
+    ncp = (int)((nsteps + seg - 1) / seg);
+    c->rec_steps = 0; // a new recording replaces the last one, also when it fails
+    // (re)allocate snapshots: the new ones are checked against the free
+    // device memory first, so an impossible request fails at once; a failure
+    // part-way (memory taken meanwhile) leaves no snapshots behind
     if ((int)c->cp_state.size() != ncp) {
         for (auto &p : c->cp_field) dfree(p);
         for (auto &p : c->cp_state) dfree(p);
+        c->cp_field.clear();
+        c->cp_state.clear();
+        const double need = (double)ncp * (2.0 * (double)c->esz * (double)c->N +
+                                           2.0 * (double)(state_bytes(c, 0) + state_bytes(c, 2)));
+        size_t free_b = 0, total_b = 0;
+        HIPC(c, hipMemGetInfo(&free_b, &total_b));
+        if (need > (double)free_b)
+            return fail(c, VIRS_ERR_NOMEM, "%d checkpoints need %.1f GB, %.1f GB of device memory are free",
+                        ncp, need / 1e9, (double)free_b / 1e9);
         c->cp_field.assign((size_t)ncp * 2, nullptr);
         c->cp_state.assign((size_t)ncp, nullptr);
         for (int k = 0; k < ncp && !rc; ++k) {
@@ -1756,8 +1768,15 @@ int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
             if (!rc) rc = dalloc(c, &c->cp_field[2 * k + 1], c->esz * (size_t)c->N);
             if (!rc) rc = dalloc(c, &c->cp_state[k], state_bytes(c, 0) * 2 + state_bytes(c, 2) * 2);
         }
-        if (rc) return rc;
+        if (rc) {
+            for (auto &p : c->cp_field) dfree(p);
+            for (auto &p : c->cp_state) dfree(p);
+            c->cp_field.clear();
+            c->cp_state.clear();
+            return rc;
+        }
     }
+    c->seg = seg;
     c->ncp = ncp;
     rc = ensure_trace(c, c->seg + 2, ncp == 1 ? 1 : 2);
     if (rc) return rc;
