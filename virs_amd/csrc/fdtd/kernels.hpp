@@ -477,8 +477,6 @@ struct DenseParams {
     BoundaryState bs;
 };
 
-// Filter taps of entry e before step n: x1 = d^{n-1}, x2 = d^{n-2},
-// y1 = y^{n-1}, y2 = y^{n-2}.
 // Element type of the filter-output ring for field storage T: FP64, or
 // FP32 for the 16-bit formats, whose storage rounding is far coarser than
 // FP32's (the reference keeps all of its filter state in float).  The
@@ -496,23 +494,12 @@ struct YsOf<bf16> {
     using type = float;
 };
 
-template <typename C, typename Y>
-__device__ inline void load_taps(const BoundaryState &bs, long long e, long long n, double &x1, double &x2,
-                                 double &y1, double &y2)
-{
-    const int s = (int)(n & 1);
-    x1 = (double)((const C *)bs.xs[s ^ 1])[e];
-    x2 = (double)((const C *)bs.xs[s])[e];
-    y1 = (double)((const Y *)bs.ys[s ^ 1])[e];
-    y2 = (double)((const Y *)bs.ys[s])[e];
-}
-
-// DF1 biquad step and boundary correction for list entry e (simulator_gpu.cu:126-148):
+// DF1 biquad step and boundary correction for one list entry (simulator_gpu.cu:126-148),
+// with x1 = d^{n-1}, x2 = d^{n-2}, y1 = y^{n-1}, y2 = y^{n-2}:
 //   y = b0 d + b1 x1 + b2 x2 - a1 y1 - a2 y2   (in double; stored as Y)
 //   R = R - K (0.1 y + 0.9 d) 0.4f             (evaluated in double)
-// x1..y2 is the entry's state as loaded before the call.
-// The filter output and the corrected pressure, the arithmetic shared by the
-// list launch and the fused stencil phase (one expression order for both).
+// The arithmetic shared by the list launch and the stencil's boundary phase
+// (one expression order for both).
 __device__ inline double biquad_out(const double *cf, double in, double x1, double x2, double y1, double y2)
 {
     return cf[0] * in + cf[1] * x1 + cf[2] * x2 - cf[3] * y1 - cf[4] * y2;
@@ -537,25 +524,6 @@ __device__ inline void store_taps(const BoundaryState &bs, long long e, long lon
             ((C *)bs.trace_y)[slot * bs.nb + e] = (C)yv;
         }
     }
-}
-
-template <typename C, typename Y>
-__device__ inline C boundary_update_pre(const BoundaryState &bs, long long e, long long n, C dif, C r,
-                                        double x1, double x2, double y1, double y2)
-{
-    const int m = bs.bmat ? bs.bmat[e] : 0;
-    const double in = (double)dif;
-    const double yv = biquad_out(bs.coeffs + 5 * m, in, x1, x2, y1, y2);
-    store_taps<C, Y>(bs, e, n, dif, yv);
-    return corrected<C>(r, bs.kconst, yv, in);
-}
-
-template <typename C, typename Y>
-__device__ inline C boundary_update(const BoundaryState &bs, long long e, long long n, C dif, C r)
-{
-    double x1, x2, y1, y2;
-    load_taps<C, Y>(bs, e, n, x1, x2, y1, y2);
-    return boundary_update_pre<C, Y>(bs, e, n, dif, r, x1, x2, y1, y2);
 }
 
 // Leapfrog 7-point update, simulator_gpu.cu:38-103 with a graded sponge:
