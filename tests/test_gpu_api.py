@@ -8,11 +8,14 @@
 - virs_time_steps times exactly the launches virs_step makes (graphs of any
   length), after an untimed sampling pass;
 - checkpoints beyond the device memory fail with an error code, and the
-  context stays usable."""
+  context stays usable;
+- the RMS stop, held off while timed, recorded and adjoint steps run, is in
+  effect again afterwards."""
 import numpy as np
 import pytest
 
 from helpers import gpu_context, tiny_problem
+from oracle import oracle as O
 from virs_amd import grid as G
 from virs_amd.fdtd import FDTD, FDTDError
 
@@ -180,4 +183,48 @@ def test_checkpoints_beyond_device_memory_fail_with_an_error(gpu):
     f.forward_record(40, 4)
     g = f.adjoint(np.ones((1, 40), np.float32))
     assert np.isfinite(g).all()
+    f.close()
+
+
+def test_rms_stop_is_restored_after_timing_recording_and_adjoint(gpu):
+    """virs_time_steps, virs_forward_record and virs_adjoint run every step
+    with the RMS rule (simulator_gpu.cu:303-330) off; a plain run afterwards
+    stops at the oracle's step again."""
+    p = tiny_problem(2, dims=(28, 20, 16))
+    flags = p["flags"]
+    nz, ny, nx = flags.shape
+    h, steps = 0.05, 3400
+    dt = 0.5 * h / (343.0 * np.sqrt(3.0))
+    row = G.rbj_highpass(np.float32(1.0 / dt), 440.0, 0.1, 1.0).astype(np.float64)[None]
+    ridx = ((nz // 2) * ny + ny // 2 + 2) * nx + nx // 2
+    bidx = G.boundary_list(flags)
+    ir = O.Problem(flags, h, bidx=bidx, coeffs=row, rcv_idx=[ridx]).run(steps, "f32")["ir"][0]
+    stop = O.rms_continue_sequence(ir.astype(np.float32))
+    assert 3000 < stop < steps - 100
+    f = FDTD((nx, ny, nz), h)
+    f.upload_flags(flags)
+    f.upload_boundary(bidx)
+    f.set_materials(row)
+    f.set_receivers([ridx])
+    f.set_early_stop(True)
+
+    def stops_at_oracle_step(after):
+        _say(f"RMS stop after {after}")
+        f.start(steps)
+        assert f.step(steps) is False
+        assert f.steps_done == stop + 1
+
+    stops_at_oracle_step("nothing")
+    f.start(steps)
+    t = f.time_steps(steps - 8, 8)  # past the stopping step, all run
+    assert t["sampled_steps"] == 8 and f.steps_done == steps
+    stops_at_oracle_step("timed steps")
+    f.start(steps)
+    f.forward_record(steps, 4)
+    assert f.steps_done == steps
+    stops_at_oracle_step("a recording")
+    f.start(steps)
+    f.forward_record(steps, 4)
+    assert np.isfinite(f.adjoint(np.ones((1, steps), np.float32))).all()
+    stops_at_oracle_step("an adjoint")
     f.close()

@@ -626,9 +626,8 @@ bool is16(const virs_ctx *c) { return c->d.precision == VIRS_FP16 || c->d.precis
 // cell codes: a byte per cell, half a byte in the 16-bit formats
 size_t code_bytes(const virs_ctx *c) { return is16(c) ? (size_t)(c->N + 1) / 2 : (size_t)c->N; }
 
-// bytes of filter-state ring k (0, 1: inputs in the compute type; 2, 3: FP64 outputs)
-// filter-state rings: inputs in the compute type, outputs in FP64 (FP32
-// for 16-bit storage, YsOf)
+// bytes of filter-state ring k: 0 and 1 hold inputs in the compute type, 2
+// and 3 outputs in FP64 (FP32 for 16-bit storage, YsOf)
 size_t state_bytes(const virs_ctx *c, int k)
 {
     const size_t ys = is16(c) ? sizeof(float) : sizeof(double);
@@ -1087,6 +1086,36 @@ static int device_early_stop(virs_ctx *c, bool on)
     HIPC(c, hipGetLastError());
     return VIRS_OK;
 }
+
+// Holds the RMS stop off while every step must run (timing, recording, the
+// adjoint's recomputation).  resume() restores the context's setting; an
+// error return before it restores the setting as well, after the work queued
+// on `other` (a second stream whose steps read the setting) has finished.
+struct EarlyStopPause {
+    virs_ctx *c;
+    hipStream_t other;
+    bool paused = false;
+    explicit EarlyStopPause(virs_ctx *ctx, hipStream_t o = nullptr) : c(ctx), other(o) {}
+    int pause()
+    {
+        const int rc = device_early_stop(c, false);
+        paused = rc == VIRS_OK;
+        return rc;
+    }
+    int resume()
+    {
+        paused = false;
+        return device_early_stop(c, true);
+    }
+    ~EarlyStopPause()
+    {
+        if (!paused) return;
+        if (other) (void)hipStreamSynchronize(other);
+        (void)device_early_stop(c, true);
+    }
+    EarlyStopPause(const EarlyStopPause &) = delete;
+    EarlyStopPause &operator=(const EarlyStopPause &) = delete;
+};
 
 int virs_start_simulation(virs_ctx *c, int64_t nsteps)
 {
@@ -1564,15 +1593,17 @@ int virs_time_steps(virs_ctx *c, int64_t nsteps, int64_t nsample, virs_timing *o
     if (rc) return rc;
     if (nsteps <= 0 || nsample < 0 || !out) return VIRS_ERR_ARG;
     std::memset(out, 0, sizeof *out);
-    rc = device_early_stop(c, false); // every sampled or timed step does its work
+    EarlyStopPause es(c); // every sampled or timed step does its work
+    rc = es.pause();
     if (rc) return rc;
     // Sampling pass: nsample single steps whose two launches record their
     // own start and stop (hipExtLaunchKernel), outside the timed region.
     std::vector<hipEvent_t> ev((size_t)nsample * 4, nullptr);
-    for (auto &e : ev) HIPC(c, hipEventCreate(&e));
+    for (auto &e : ev)
+        if (!rc && hipEventCreate(&e) != hipSuccess) rc = fail(c, VIRS_ERR_HIP, "event create");
     for (long long k = 0; k < nsample && !rc; ++k) {
         rc = launch_step(c, nullptr, nullptr, 0, true, &ev[(size_t)k * 4]);
-        ++c->steps;
+        if (!rc) ++c->steps;
     }
     // The timed region runs exactly what virs_step runs; its graphs are
     // built before it starts.
@@ -1585,7 +1616,7 @@ int virs_time_steps(virs_ctx *c, int64_t nsteps, int64_t nsample, virs_timing *o
     if (!rc) rc = run_steps(c, nsteps);
     if (!rc && hipEventRecord(t1, c->stream) != hipSuccess) rc = fail(c, VIRS_ERR_HIP, "event record");
     c->counted = c->steps;
-    if (!rc) rc = device_early_stop(c, true);
+    if (!rc) rc = es.resume();
     if (!rc && hipEventSynchronize(t1) != hipSuccess) rc = fail(c, VIRS_ERR_HIP, "event sync");
     double dsum = 0, bsum = 0;
     for (long long j = 0; j < nsample && !rc; ++j) {
@@ -1780,7 +1811,8 @@ int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
     c->ncp = ncp;
     rc = ensure_trace(c, c->seg + 2, ncp == 1 ? 1 : 2);
     if (rc) return rc;
-    rc = device_early_stop(c, false);
+    EarlyStopPause es(c);
+    rc = es.pause();
     if (!rc) rc = reset_run(c);
     if (rc) return rc;
     const long long last_base = (long long)(ncp - 1) * c->seg;
@@ -1802,7 +1834,7 @@ int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
     }
     c->counted = c->steps;
     c->rec_steps = nsteps;
-    rc = device_early_stop(c, true);
+    rc = es.resume();
     if (rc) return rc;
     HIPC(c, hipStreamSynchronize(c->stream));
     return VIRS_OK;
@@ -1881,7 +1913,8 @@ int virs_adjoint(virs_ctx *c, const float *dl_dir, double *grad)
         if (c->nb) HIPC(c, hipMemsetAsync(c->ast[k], 0, sizeof(double) * c->nb, c->stream));
     hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl_adj, 0, T - 1, 1);
     HIPC(c, hipGetLastError());
-    rc = device_early_stop(c, false);
+    EarlyStopPause es(c, c->stream2);
+    rc = es.pause();
     if (rc) return rc;
     // the second stream starts after everything queued so far
     HIPC(c, hipStreamSynchronize(c->stream));
@@ -1921,7 +1954,7 @@ int virs_adjoint(virs_ctx *c, const float *dl_dir, double *grad)
     // and the forward state is the finished run's.
     c->rec_steps = 0;
     c->restart_needed = c->ncp >= 2;
-    rc = device_early_stop(c, true);
+    rc = es.resume();
     if (rc) return rc;
     HIPC(c, hipStreamSynchronize(c->stream));
     return VIRS_OK;
