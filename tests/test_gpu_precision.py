@@ -50,6 +50,35 @@ def test_16bit_storage_bit_exact(gpu, fmt, rounding, seed, hard):
     f.close()
 
 
+def test_fp16_subnormal_neighbours_bit_exact(gpu):
+    """Early in a run the wavefront's leading cells hold values far below the
+    field's maximum, so some stored FP16 values are subnormal.  The stencil
+    adds its y-neighbours to the sum straight from the stored pairs; the
+    step after such a field still equals the oracle's bit for bit.  The test
+    first checks that the oracle's field at that point does hold subnormal
+    stored values next to updated cells."""
+    p = tiny_problem(31, steps=70, hard=False)
+    n = 10
+    ref = oracle_problem(p).run16(n, "fp16", True, want_fields=True)
+    raw = ref["p"] * 2.0 ** int(ref["exp"][-1])
+    sub = (raw != 0) & (np.abs(raw) < 2.0 ** -14)
+    upd = p["flags"] != 1
+    next_to = np.zeros_like(sub)
+    next_to[:, 1:, :] |= sub[:, :-1, :] & upd[:, 1:, :]
+    next_to[:, :-1, :] |= sub[:, 1:, :] & upd[:, :-1, :]
+    assert next_to.sum() >= 5, "no subnormal stored value is a y-neighbour of an updated cell"
+    after = oracle_problem(p).run16(n + 1, "fp16", True, want_fields=True)
+    f = gpu_context(p, "fp16")
+    f.step(n)
+    np.testing.assert_array_equal(f.field(0), ref["p"])
+    f.step(1)
+    f.synchronize()
+    np.testing.assert_array_equal(f.field(0), after["p"])
+    np.testing.assert_array_equal(f.field(1), after["q"])
+    np.testing.assert_array_equal(f.receivers(True), after["ir"])
+    f.close()
+
+
 @pytest.mark.parametrize("fmt", ["fp16", "bf16"])
 @pytest.mark.parametrize("nx", [23, 26, 28, 40])
 def test_16bit_odd_widths(gpu, fmt, nx):

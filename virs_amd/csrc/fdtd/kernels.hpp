@@ -236,6 +236,20 @@ __device__ inline void vstore(VT *p, const VT &v)
 
 __device__ inline float exp2i(int e) { return __int_as_float((e + 127) << 23); }
 
+// sum + h, h the low (HI false) or high FP16 half of w, as one
+// v_fma_mix_f32 computing 1*h + sum: the product is exact, so the sum is
+// rounded once, bit for bit the result of converting h and adding it
+template <bool HI>
+__device__ inline float add_f16_half(float sum, uint32_t w)
+{
+    float r;
+    if constexpr (HI)
+        asm("v_fma_mix_f32 %0, 1.0, %1, %2 op_sel:[0,1,0] op_sel_hi:[0,1,0]" : "=v"(r) : "v"(w), "v"(sum));
+    else
+        asm("v_fma_mix_f32 %0, 1.0, %1, %2 op_sel_hi:[0,1,0]" : "=v"(r) : "v"(w), "v"(sum));
+    return r;
+}
+
 // Orders a wave's LDS accesses around a lane-to-lane exchange inside the
 // wave: the hardware performs one wave's LDS operations in issue order, so
 // only the compiler's ordering needs fixing, which a wavefront-scope fence
@@ -787,6 +801,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                 kwo = (cur.cd >> 4) & 0x07070707u;
                 asm volatile("" : "+v"(kwe), "+v"(kwo));
             }
+            // FP16: the y-neighbours enter the sum straight from their
+            // stored pairs (add_f16_half), without a conversion each
+            constexpr bool kMixY = std::is_same<T, f16>::value && V >= 2;
+            uint32_t ynw[kMixY ? V / 2 : 1], ysw[kMixY ? V / 2 : 1];
+            if constexpr (kMixY) {
+                __builtin_memcpy(ynw, &cur.yn, sizeof(VT));
+                __builtin_memcpy(ysw, &cur.ys, sizeof(VT));
+            }
 #pragma unroll
             for (int j = 0; j < V; ++j) {
                 // at z = nz-1 this is plane z itself; that plane is border,
@@ -820,8 +842,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                 const C pcv = PC[j];
                 if constexpr (S::kScaled) {
                     C sum = (j + 1 < V ? PC[(j + 1) % V] : right) + (j > 0 ? PC[(j + V - 1) % V] : left);
-                    sum = sum + yn[j];
-                    sum = sum + ys[j];
+                    if constexpr (kMixY) {
+                        if (j & 1) {
+                            sum = add_f16_half<true>(sum, ynw[j >> 1]);
+                            sum = add_f16_half<true>(sum, ysw[j >> 1]);
+                        } else {
+                            sum = add_f16_half<false>(sum, ynw[j >> 1]);
+                            sum = add_f16_half<false>(sum, ysw[j >> 1]);
+                        }
+                    } else {
+                        sum = sum + yn[j];
+                        sum = sum + ys[j];
+                    }
                     sum = sum + PP[j];
                     sum = sum + PM[j];
                     const C kw = (C)(((j & 1) ? kwo : kwe) >> (8 * (j >> 1)) & 0xffu); // cells that update
