@@ -65,9 +65,11 @@ typedef struct {
     int32_t nrcv;
     const int64_t *rcv_idx;         /* [nrcv] */
     const double *init_p, *init_q;  /* optional initial p^0 and p^-1, NULL = zero */
-    double *state_io;               /* optional [nb][4] filter state x1 x2 y1 y2: read at the
-                                     * start of an FP32/FP64 run, written at its end, so a run
-                                     * can continue in chunks (with init_p/init_q and n0) */
+    double *state_io;               /* optional [nb][4] filter state, read at the start of an
+                                     * FP32/FP64 run and written at its end, so a run can
+                                     * continue in chunks (with init_p/init_q and n0): x1 x2 y1 y2
+                                     * of the float direct form I (state_f32), or w1 w2 0 0 of
+                                     * the double direct form II */
 } oracle_desc;
 
 static int g_threads = 1;
@@ -102,8 +104,13 @@ static int sponge_dist(const oracle_desc *d, int x, int y, int z)
  * V(n) = pc if flags[n]==1 else P[n]                     (simulator_gpu.cu:70-81)
  * The boundary correction is evaluated in double as the reference's literal
  * 0.1/0.9 promote it (simulator_gpu.cu:147-148).
+ * The boundary biquad (simulator_gpu.cu:130-145) runs either as the
+ * reference writes it, direct form I with float state and float
+ * coefficients (state_f32), or as the GPU path runs it for FP32/FP64
+ * fields, direct form II with double state; both forms are the same filter
+ * in exact arithmetic.
  */
-#define DEFINE_RUNNER(T, SUFFIX)                                                          \
+#define DEFINE_RUNNER(T, SUFFIX)                                                        \
 int oracle_run_##SUFFIX(const oracle_desc *d, int64_t n0, int64_t nsteps, double *ir,      \
                         double *final_p, double *final_q, double *trace_d,                 \
                         double *trace_y)                                                   \
@@ -178,11 +185,13 @@ int oracle_run_##SUFFIX(const oracle_desc *d, int64_t n0, int64_t nsteps, double
                 s4[1] = s4[0]; s4[0] = in; s4[3] = s4[2]; s4[2] = o;                      \
                 yv = (double)o;                                                           \
             } else {                                                                      \
+                /* direct form II in double: w = d - a1 w1 - a2 w2,             */       \
+                /* y = b0 w + b1 w1 + b2 w2, state (w1, w2) in s4[0..1]         */       \
                 double *s4 = st + 4 * e;                                                  \
                 const double in = (double)dif;                                            \
-                const double o = c[0] * in + c[1] * s4[0] + c[2] * s4[1]                  \
-                                 - c[3] * s4[2] - c[4] * s4[3];                           \
-                s4[1] = s4[0]; s4[0] = in; s4[3] = s4[2]; s4[2] = o;                      \
+                const double w = in - c[3] * s4[0] - c[4] * s4[1];                        \
+                const double o = c[0] * w + c[1] * s4[0] + c[2] * s4[1];                  \
+                s4[1] = s4[0]; s4[0] = w;                                                 \
                 yv = o;                                                                   \
             }                                                                             \
             if (trace_d) trace_d[s * d->nb + e] = (double)dif;                            \

@@ -5,7 +5,8 @@ Each check is chosen so a plausible slip in the restatement fails it:
   eigenmodes      wrong wall-mirror rule, wrong Courant constant, wrong sign
   energy          asymmetric neighbour handling
   reciprocity     a boundary correction that couples cells
-  lfilter         DF1 recursion order / coefficient order / sign convention
+  lfilter         filter recursion order / coefficient order / sign convention,
+                  in both forms (float direct form I, double direct form II)
   single cell     the 0.1/0.9/0.4/K correction constants
   numpy float32   operand order and indexing of the FP32 dense update
   finite diffs    the hand-written adjoint
@@ -101,9 +102,21 @@ def test_reciprocity_with_boundary_filters():
 
 
 @pytest.mark.parametrize("state_f32", [False, True])
-def test_df1_matches_lfilter(state_f32):
-    p = tiny_problem(12, steps=80)
-    out = oracle_problem(p, state_f32=state_f32).run(80, "f64" if not state_f32 else "f32",
+@pytest.mark.parametrize("rbj", [False, True])
+def test_boundary_filter_matches_lfilter(state_f32, rbj):
+    """Every list entry's filter output over the run equals scipy's lfilter
+    of its input sequence: the reference's float direct form I (state_f32)
+    and the double direct form II of the FP32/FP64 GPU path.  rbj: the
+    reference's default high-pass row (440 Hz, Q 0.1, simulator.cpp:955-967),
+    whose real pole at 0.99909 makes the form II state about 1e4 times its
+    input at low frequencies."""
+    p = tiny_problem(12, steps=400 if rbj else 80)
+    if rbj:
+        from virs_amd import grid as G
+        row = G.rbj_highpass(88200.0 * 2.0 * np.sqrt(3.0), 440.0, 0.1)
+        p["coeffs"] = np.tile(np.asarray(row, np.float64), (len(p["coeffs"]), 1))
+    nst = p["steps"]
+    out = oracle_problem(p, state_f32=state_f32).run(nst, "f64" if not state_f32 else "f32",
                                                    want_trace=True)
     td, ty = out["trace_d"], out["trace_y"]
     tol = 1e-12 if not state_f32 else 2e-5

@@ -444,10 +444,17 @@ struct BoundaryState {
     // Filter state as two-slot rings indexed by step parity: on entry to
     // step n, slot n&1 holds the value of step n-2 and slot (n+1)&1 that of
     // step n-1; step n overwrites slot n&1, so a step writes one value per
-    // ring instead of shifting the state.  Forward: xs = filter inputs d in
-    // the field's compute type C (d is a difference of two C values, so C
-    // holds it exactly), ys = outputs y in FP64, or FP32 for 16-bit storage
-    // (YsOf).  Reverse: xs = dbar, ys = ybar, both FP64.
+    // ring instead of shifting the state.
+    // FP32/FP64 fields: the direct-form-II state w in FP64 (ws), 8 bytes
+    // read twice and written once per entry and step.  xs/ys then hold the
+    // history of the filter inputs d (in the compute type C, which holds d
+    // exactly: a difference of two C values) and outputs y (FP64): written
+    // only when non-null (the recording forward's last two steps before
+    // each snapshot, which seed the adjoint's trace), never read by a step.
+    // 16-bit fields: direct form I, xs = inputs d in FP32 and ys = outputs
+    // y stored in FP32 (YsOf), read and written every step; ws unused.
+    // Reverse: xs = dbar, ys = ybar, both FP64.
+    void *ws[2];
     void *xs[2];
     void *ys[2];
     const double *coeffs;      // [nmat][5] b0 b1 b2 a1 a2
@@ -508,9 +515,15 @@ struct YsOf<bf16> {
     using type = float;
 };
 
-// DF1 biquad step and boundary correction for one list entry (simulator_gpu.cu:126-148),
-// with x1 = d^{n-1}, x2 = d^{n-2}, y1 = y^{n-1}, y2 = y^{n-2}:
-//   y = b0 d + b1 x1 + b2 x2 - a1 y1 - a2 y2   (in double; stored as Y)
+// Biquad step and boundary correction for one list entry (simulator_gpu.cu:126-148).
+// The reference's filter is y = b0 d + b1 d^{n-1} + b2 d^{n-2} - a1 y^{n-1} - a2 y^{n-2}
+// with float state.  The same filter is run in double in one of two forms:
+//   direct form I (16-bit fields), x1 = d^{n-1}, x2 = d^{n-2}, y1 = y^{n-1}, y2 = y^{n-2}:
+//     y = b0 d + b1 x1 + b2 x2 - a1 y1 - a2 y2
+//   direct form II (FP32/FP64 fields), w1 = w^{n-1}, w2 = w^{n-2}:
+//     w = d - a1 w1 - a2 w2,   y = b0 w + b1 w1 + b2 w2
+// Both give the same y in exact arithmetic; form II keeps two FP64 values
+// of state per entry instead of two inputs and two outputs.  Then
 //   R = R - K (0.1 y + 0.9 d) 0.4f             (evaluated in double)
 // The arithmetic shared by the list launch and the stencil's boundary phase
 // (one expression order for both).
@@ -518,19 +531,37 @@ __device__ inline double biquad_out(const double *cf, double in, double x1, doub
 {
     return cf[0] * in + cf[1] * x1 + cf[2] * x2 - cf[3] * y1 - cf[4] * y2;
 }
+__device__ inline double df2_state(const double *cf, double in, double w1, double w2)
+{
+    return in - cf[3] * w1 - cf[4] * w2;
+}
+__device__ inline double df2_out(const double *cf, double w, double w1, double w2)
+{
+    return cf[0] * w + cf[1] * w1 + cf[2] * w2;
+}
 template <typename C>
 __device__ inline C corrected(C r, float kconst, double yv, double in)
 {
     return (C)((double)r - (double)kconst * (yv * 0.1 + in * 0.9) * (double)0.4f);
 }
 
-// Stores of entry e at step n: the new filter taps and, when recording,
-// the trace of the adjoint's segment.
-template <typename C, typename Y>
-__device__ inline void store_taps(const BoundaryState &bs, long long e, long long n, C dif, double yv)
+// Stores of entry e at step n: the new filter state (FP64 w for FP32/FP64
+// fields, with the d/y history when its rings are given; d and y for 16-bit
+// fields) and, when recording, the trace of the adjoint's segment.
+template <typename C, typename Y, bool DF2>
+__device__ inline void store_taps(const BoundaryState &bs, long long e, long long n, C dif, double yv,
+                                  double wv)
 {
-    ((C *)bs.xs[n & 1])[e] = dif;
-    ((Y *)bs.ys[n & 1])[e] = (Y)yv;
+    if constexpr (DF2) {
+        ((double *)bs.ws[n & 1])[e] = wv;
+        if (bs.xs[0]) {
+            ((C *)bs.xs[n & 1])[e] = dif;
+            ((Y *)bs.ys[n & 1])[e] = (Y)yv;
+        }
+    } else {
+        ((C *)bs.xs[n & 1])[e] = dif;
+        ((Y *)bs.ys[n & 1])[e] = (Y)yv;
+    }
     if (bs.trace_d) {
         const long long slot = n - bs.trace_base + 2;
         if (slot >= 0 && slot < bs.trace_slots) {
@@ -669,8 +700,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
         // filter-input and filter-output rings (lanes past the count hold
         // the slice's last entry)
         uint32_t efirst, ecount, eb, em;
-        C ex0, ex1;
+        C ex0, ex1;  // direct form I (16-bit fields)
         Y ey0, ey1;
+        double ew0, ew1; // direct form II (FP32/FP64 fields)
     };
     // FUSED: the cells' (d, stored p^{n+1}) of one wave and plane, by
     // position j in the lane's vector and lane, so an entry's lane can read
@@ -766,10 +798,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                 const long long ec = min((long long)st.efirst + k, p.bs.nb - 1);
                 st.eb = p.bidx[ec];
                 st.em = p.bs.bmat[ec];
-                st.ex0 = ((const C *)p.bs.xs[0])[ec];
-                st.ex1 = ((const C *)p.bs.xs[1])[ec];
-                st.ey0 = ((const Y *)p.bs.ys[0])[ec];
-                st.ey1 = ((const Y *)p.bs.ys[1])[ec];
+                if constexpr (S::kScaled) {
+                    st.ex0 = ((const C *)p.bs.xs[0])[ec];
+                    st.ex1 = ((const C *)p.bs.xs[1])[ec];
+                    st.ey0 = ((const Y *)p.bs.ys[0])[ec];
+                    st.ey1 = ((const Y *)p.bs.ys[1])[ec];
+                } else {
+                    st.ew0 = ((const double *)p.bs.ws[0])[ec];
+                    st.ew1 = ((const double *)p.bs.ws[1])[ec];
+                }
             }
         };
 
@@ -990,28 +1027,42 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                         uint32_t b = cur.eb, m = cur.em;
                         C xa = cur.ex0, xb = cur.ex1;
                         Y ya = cur.ey0, yb = cur.ey1;
+                        double wa = cur.ew0, wb = cur.ew1;
                         const long long e = (long long)cur.efirst + r0 + lane;
                         if (r0 > 0) { // entries past the first 64 (floor and ceiling planes)
                             const long long ec = min(e, (long long)cur.efirst + cnt - 1);
                             b = p.bidx[ec];
                             m = p.bs.bmat[ec];
-                            xa = ((const C *)p.bs.xs[0])[ec];
-                            xb = ((const C *)p.bs.xs[1])[ec];
-                            ya = ((const Y *)p.bs.ys[0])[ec];
-                            yb = ((const Y *)p.bs.ys[1])[ec];
+                            if constexpr (S::kScaled) {
+                                xa = ((const C *)p.bs.xs[0])[ec];
+                                xb = ((const C *)p.bs.xs[1])[ec];
+                                ya = ((const Y *)p.bs.ys[0])[ec];
+                                yb = ((const Y *)p.bs.ys[1])[ec];
+                            } else {
+                                wa = ((const double *)p.bs.ws[0])[ec];
+                                wb = ((const double *)p.bs.ws[1])[ec];
+                            }
                         }
                         if ((uint32_t)lane < cnt - r0) {
                             const uint32_t off = b - pbase;
                             BSlot &sl = bslot[wave][off % (uint32_t)V][off / (uint32_t)V];
                             const C dif = sl.d;
                             const double in = (double)dif;
-                            const double x1 = (double)(odd ? xa : xb), x2 = (double)(odd ? xb : xa);
-                            const double y1 = (double)(odd ? ya : yb), y2 = (double)(odd ? yb : ya);
-                            const double yv = biquad_out(cf_lds + 5 * m, in, x1, x2, y1, y2);
+                            const double *cf = cf_lds + 5 * m;
+                            double yv, wv = 0.0;
+                            if constexpr (S::kScaled) {
+                                const double x1 = (double)(odd ? xa : xb), x2 = (double)(odd ? xb : xa);
+                                const double y1 = (double)(odd ? ya : yb), y2 = (double)(odd ? yb : ya);
+                                yv = biquad_out(cf, in, x1, x2, y1, y2);
+                            } else {
+                                const double w1 = odd ? wa : wb, w2 = odd ? wb : wa;
+                                wv = df2_state(cf, in, w1, w2);
+                                yv = df2_out(cf, wv, w1, w2);
+                            }
                             const C r = corrected<C>(sl.r * kst_inv, p.bs.kconst, yv, in);
                             flag_nonfinite(p.ctl, r);
                             const T t = store_round<T>(r * kst, SR, (unsigned long long)b, n, kSaltBoundary);
-                            store_taps<C, Y>(p.bs, e, n, dif, yv);
+                            store_taps<C, Y, !S::kScaled>(p.bs, e, n, dif, yv, wv);
                             sl.r = S::ld(t);
                         }
                     }
@@ -1189,8 +1240,12 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
         uint32_t bv[U];
         uint8_t mv[U];
         T h0[U], h1[U], rt[U];
+        // filter state: direct form II (FP64 w) for FP32/FP64 fields, direct
+        // form I (FP32 inputs and outputs) for 16-bit fields
+        constexpr bool kDF2 = !S::kScaled;
         C xa[U], xb[U];
         Y ya[U], yb[U];
+        double wa[U], wb[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             ev[u] = ((long long)(blockIdx.x - 1) * U + u) * kTailBlock + threadIdx.x;
@@ -1199,10 +1254,15 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
             mv[u] = p.bs.bmat[ec]; // always present for the list launch
             h0[u] = ((const T *)p.hv[0])[ec];
             h1[u] = ((const T *)p.hv[1])[ec];
-            xa[u] = ((const C *)p.bs.xs[0])[ec];
-            xb[u] = ((const C *)p.bs.xs[1])[ec];
-            ya[u] = ((const Y *)p.bs.ys[0])[ec];
-            yb[u] = ((const Y *)p.bs.ys[1])[ec];
+            if constexpr (kDF2) {
+                wa[u] = ((const double *)p.bs.ws[0])[ec];
+                wb[u] = ((const double *)p.bs.ws[1])[ec];
+            } else {
+                xa[u] = ((const C *)p.bs.xs[0])[ec];
+                xb[u] = ((const C *)p.bs.xs[1])[ec];
+                ya[u] = ((const Y *)p.bs.ys[0])[ec];
+                yb[u] = ((const Y *)p.bs.ys[1])[ec];
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) rt[u] = R[bv[u]];
@@ -1216,17 +1276,24 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
             const uint32_t b = bv[u];
             const double *cf = p.bs.coeffs + 5 * mv[u];
             C pp = S::ld(odd ? h1[u] : h0[u]), qq = S::ld(odd ? h0[u] : h1[u]), r = S::ld(rt[u]);
-            const double x1 = (double)(odd ? xa[u] : xb[u]), x2 = (double)(odd ? xb[u] : xa[u]);
-            const double y1 = (double)(odd ? ya[u] : yb[u]), y2 = (double)(odd ? yb[u] : ya[u]);
             if constexpr (S::kScaled) pp = pp * sP, qq = qq * sQ, r = r * sRi;
             const C dif = pp - qq;
             const double in = (double)dif;
-            const double yv = biquad_out(cf, in, x1, x2, y1, y2);
+            double yv, wv = 0.0;
+            if constexpr (kDF2) {
+                const double w1 = odd ? wa[u] : wb[u], w2 = odd ? wb[u] : wa[u];
+                wv = df2_state(cf, in, w1, w2);
+                yv = df2_out(cf, wv, w1, w2);
+            } else {
+                const double x1 = (double)(odd ? xa[u] : xb[u]), x2 = (double)(odd ? xb[u] : xa[u]);
+                const double y1 = (double)(odd ? ya[u] : yb[u]), y2 = (double)(odd ? yb[u] : ya[u]);
+                yv = biquad_out(cf, in, x1, x2, y1, y2);
+            }
             r = corrected<C>(r, p.bs.kconst, yv, in);
             T t = store_round<T>(r * sR, p.sr, (unsigned long long)b, n, kSaltBoundary);
             if (e < nb && !skip) {
                 flag_nonfinite(p.ctl, r);
-                store_taps<C, Y>(p.bs, e, n, dif, yv);
+                store_taps<C, Y, kDF2>(p.bs, e, n, dif, yv, wv);
                 // a soft source on this cell is added to the stored corrected
                 // value (the point tables are scanned only when some point is
                 // on the list)

@@ -61,9 +61,12 @@ struct virs_ctx {
     long long nb = 0;
     uint32_t *bidx = nullptr;
     uint8_t *bmat = nullptr;
-    // filter state rings (BoundaryState::xs, ys): st[0..1] filter inputs in
-    // the compute type (4 or 8 bytes), st[2..3] outputs in FP64
-    void *st[4] = {nullptr, nullptr, nullptr, nullptr};
+    // filter state rings (BoundaryState): st[0..1] filter inputs d in the
+    // compute type (xs), st[2..3] outputs y (ys: FP64, FP32 for 16-bit
+    // fields), st[4..5] the direct-form-II state w in FP64 (ws, FP32/FP64
+    // fields only; there xs/ys are the d/y history the adjoint's recording
+    // writes before each snapshot)
+    void *st[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     void *hv[2] = {nullptr, nullptr}; // list-cell pressures by step parity (TailParams::hv)
     std::vector<uint32_t> h_bidx;
     std::vector<uint8_t> h_bmat;
@@ -277,15 +280,26 @@ int upload_points(virs_ctx *c)
     return VIRS_OK;
 }
 
-BoundaryState boundary_state(const virs_ctx *c, void *const *st)
+bool is16(const virs_ctx *c);
+
+// The forward filter state of a step: direct form I rings for 16-bit
+// fields; the direct-form-II rings for FP32/FP64 fields, with the d/y
+// history rings only when `history` is set (BoundaryState).
+BoundaryState boundary_state(const virs_ctx *c, bool history = false)
 {
     BoundaryState b{};
     b.nb = c->nb;
     b.bmat = c->bmat;
-    b.xs[0] = st[0];
-    b.xs[1] = st[1];
-    b.ys[0] = st[2];
-    b.ys[1] = st[3];
+    if (is16(c) || history) {
+        b.xs[0] = c->st[0];
+        b.xs[1] = c->st[1];
+        b.ys[0] = c->st[2];
+        b.ys[1] = c->st[3];
+    }
+    if (!is16(c)) {
+        b.ws[0] = c->st[4];
+        b.ws[1] = c->st[5];
+    }
     b.coeffs = c->coeffs;
     b.nmat = c->nmat;
     b.kconst = c->kconst;
@@ -319,7 +333,7 @@ DenseParams dense_params(const virs_ctx *c, const void *P, const void *Q, void *
     p.sp = c->sp;
     p.sr = c->d.rounding == VIRS_ROUND_STOCHASTIC;
     p.wtab = nullptr;
-    p.bs = boundary_state(c, c->st);
+    p.bs = boundary_state(c);
     return p;
 }
 
@@ -331,7 +345,7 @@ TailParams tail_params(const virs_ctx *c, const void *P, const void *Q, void *R,
     t.Q = Q;
     t.R = R;
     t.bidx = c->bidx;
-    t.bs = boundary_state(c, c->st);
+    t.bs = boundary_state(c);
     t.ctl = ctl;
     t.slot = slot;
     t.next_slot = next_slot;
@@ -540,13 +554,16 @@ hipError_t launch_tail_adj(const virs_ctx *c, const TailParams &t)
 }
 
 // One forward step on the context's current rotation; optionally records the
-// boundary trace for the adjoint.
+// boundary trace for the adjoint, or (history) the d/y history rings that
+// seed a snapshot's trace.
 int launch_step(virs_ctx *c, void *trace_d = nullptr, void *trace_y = nullptr,
-                long long trace_base = 0, bool receivers = true, hipEvent_t *ev = nullptr)
+                long long trace_base = 0, bool receivers = true, hipEvent_t *ev = nullptr,
+                bool history = false)
 {
     DenseParams p = dense_params(c, c->f[c->ip], c->f[c->iq], c->f[c->ir], c->ctl, c->ip);
     TailParams t = tail_params(c, c->f[c->ip], c->f[c->iq], c->f[c->ir], c->ctl, c->ip, c->ir);
     BoundaryState &bs = c->fused ? p.bs : t.bs;
+    if (history) bs = boundary_state(c, true);
     bs.trace_d = trace_d;
     bs.trace_y = trace_y;
     bs.trace_base = trace_base;
@@ -626,12 +643,23 @@ bool is16(const virs_ctx *c) { return c->d.precision == VIRS_FP16 || c->d.precis
 // cell codes: a byte per cell, half a byte in the 16-bit formats
 size_t code_bytes(const virs_ctx *c) { return is16(c) ? (size_t)(c->N + 1) / 2 : (size_t)c->N; }
 
-// bytes of filter-state ring k: 0 and 1 hold inputs in the compute type, 2
-// and 3 outputs in FP64 (FP32 for 16-bit storage, YsOf)
+// bytes of filter-state ring k (virs_ctx::st): 0 and 1 hold inputs in the
+// compute type, 2 and 3 outputs in FP64 (FP32 for 16-bit storage, YsOf), 4
+// and 5 the direct-form-II state in FP64 (none for 16-bit storage)
 size_t state_bytes(const virs_ctx *c, int k)
 {
     const size_t ys = is16(c) ? sizeof(float) : sizeof(double);
-    return (k < 2 ? compute_bytes(c) : ys) * (size_t)std::max<long long>(1, c->nb);
+    const size_t el = k < 2 ? compute_bytes(c) : k < 4 ? ys : is16(c) ? 0 : sizeof(double);
+    return el * (size_t)std::max<long long>(1, c->nb);
+}
+// filter-state rings in use: six with FP32/FP64 fields, four with 16-bit
+int state_rings(const virs_ctx *c) { return is16(c) ? 4 : 6; }
+// bytes of all filter-state rings (one snapshot's share)
+size_t state_total(const virs_ctx *c)
+{
+    size_t b = 0;
+    for (int k = 0; k < state_rings(c); ++k) b += state_bytes(c, k);
+    return b;
 }
 
 size_t elem_size(int prec)
@@ -655,7 +683,7 @@ int reset_run(virs_ctx *c)
 {
     int rc = zero_fields(c, c->f);
     if (rc) return rc;
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < state_rings(c); ++k)
         if (c->st[k] && c->nb) HIPC(c, hipMemsetAsync(c->st[k], 0, state_bytes(c, k), c->stream));
     hipLaunchKernelGGL(k_ctl_set, dim3(1), dim3(64), 0, c->stream, c->ctl, 0, 0LL, 1);
     HIPC(c, hipGetLastError());
@@ -966,7 +994,7 @@ int virs_upload_boundary(virs_ctx *c, const uint32_t *idx, const uint8_t *mat, i
     if (mat) c->h_bmat.assign(mat, mat + nb);
     else c->h_bmat.clear();
     int rc = dalloc(c, &c->bidx, sizeof(uint32_t) * nb);
-    for (int k = 0; k < 4 && !rc; ++k) rc = dalloc(c, &c->st[k], state_bytes(c, k));
+    for (int k = 0; k < state_rings(c) && !rc; ++k) rc = dalloc(c, &c->st[k], state_bytes(c, k));
     for (int k = 0; k < 2 && !rc; ++k) rc = dalloc(c, &c->hv[k], c->esz * (size_t)std::max<long long>(1, nb));
     // the material array always exists on the device (zeros without one):
     // the list launch reads it unconditionally
@@ -975,7 +1003,7 @@ int virs_upload_boundary(virs_ctx *c, const uint32_t *idx, const uint8_t *mat, i
     if (nb) HIPC(c, hipMemcpy(c->bidx, idx, sizeof(uint32_t) * nb, hipMemcpyHostToDevice));
     if (nb && mat) HIPC(c, hipMemcpy(c->bmat, mat, (size_t)nb, hipMemcpyHostToDevice));
     if (nb && !mat) HIPC(c, hipMemset(c->bmat, 0, (size_t)nb));
-    for (int k = 0; k < 4; ++k)
+    for (int k = 0; k < state_rings(c); ++k)
         if (nb) HIPC(c, hipMemset(c->st[k], 0, state_bytes(c, k)));
     rc = stash_boundary(c, c->steps);
     if (rc) return rc;
@@ -1573,10 +1601,10 @@ int virs_memory_bytes(const virs_ctx *c, int64_t *bytes)
     if (!c || !bytes) return VIRS_ERR_ARG;
     long long b = 3 * (long long)c->esz * c->N + (long long)code_bytes(c); // fields + cell codes
     if (c->normals) b += c->N;
-    b += c->nb * (4 + 2 * (long long)compute_bytes(c) + 16 + 2 * (long long)c->esz + 1);
+    b += c->nb * (4 + 2 * (long long)c->esz + 1) + (long long)state_total(c);
     b += (long long)sizeof(double) * std::max(1, c->nrcv) * std::max<long long>(1, c->ir_cap);
     b += (long long)c->cp_field.size() * (long long)c->esz * c->N;
-    b += (long long)c->cp_state.size() * (2 * (long long)compute_bytes(c) + 16) * c->nb;
+    b += (long long)c->cp_state.size() * (long long)state_total(c);
     b += (c->trace_d[1] ? 4 : 2) * (long long)compute_bytes(c) * c->trace_slots * c->nb; // d and y per buffer
     for (auto p : c->af)
         if (p) b += (long long)c->esz * c->N;
@@ -1718,7 +1746,7 @@ static int save_checkpoint(virs_ctx *c, int k)
     HIPC(c, hipMemcpyAsync(c->cp_field[2 * k], c->f[c->ip], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
     HIPC(c, hipMemcpyAsync(c->cp_field[2 * k + 1], c->f[c->iq], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
     size_t off = 0;
-    for (int j = 0; j < 4 && c->nb; ++j) {
+    for (int j = 0; j < state_rings(c) && c->nb; ++j) {
         HIPC(c, hipMemcpyAsync(c->cp_state[k] + off, c->st[j], state_bytes(c, j), hipMemcpyDeviceToDevice,
                                c->stream));
         off += state_bytes(c, j);
@@ -1738,7 +1766,7 @@ static int recompute_segment(virs_ctx *c, int k)
     HIPC(c, hipMemcpyAsync(c->f[c->ip], c->cp_field[2 * k], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
     HIPC(c, hipMemcpyAsync(c->f[c->iq], c->cp_field[2 * k + 1], c->esz * c->N, hipMemcpyDeviceToDevice, c->stream));
     size_t off = 0;
-    for (int j = 0; j < 4 && c->nb; ++j) {
+    for (int j = 0; j < state_rings(c) && c->nb; ++j) {
         HIPC(c, hipMemcpyAsync(c->st[j], c->cp_state[k] + off, state_bytes(c, j), hipMemcpyDeviceToDevice,
                                c->stream));
         off += state_bytes(c, j);
@@ -1786,7 +1814,7 @@ int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
         c->cp_field.clear();
         c->cp_state.clear();
         const double need = (double)ncp * (2.0 * (double)c->esz * (double)c->N +
-                                           2.0 * (double)(state_bytes(c, 0) + state_bytes(c, 2)));
+                                           (double)state_total(c));
         size_t free_b = 0, total_b = 0;
         HIPC(c, hipMemGetInfo(&free_b, &total_b));
         if (need > (double)free_b)
@@ -1797,7 +1825,7 @@ int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
         for (int k = 0; k < ncp && !rc; ++k) {
             rc = dalloc(c, &c->cp_field[2 * k], c->esz * (size_t)c->N);
             if (!rc) rc = dalloc(c, &c->cp_field[2 * k + 1], c->esz * (size_t)c->N);
-            if (!rc) rc = dalloc(c, &c->cp_state[k], state_bytes(c, 0) * 2 + state_bytes(c, 2) * 2);
+            if (!rc) rc = dalloc(c, &c->cp_state[k], state_total(c));
         }
         if (rc) {
             for (auto &p : c->cp_field) dfree(p);
@@ -1818,10 +1846,17 @@ int virs_forward_record(virs_ctx *c, int64_t nsteps, int32_t ncp)
     const long long last_base = (long long)(ncp - 1) * c->seg;
     const int last_buf = (ncp - 1) & 1;
     // segments before the last: a checkpoint, then plain steps (graphed like
-    // virs_step's); the last segment records its boundary trace step by step
+    // virs_step's), the last two of them writing the filter's d/y history,
+    // which the next snapshot keeps for the first steps of its trace; the
+    // last segment records its boundary trace step by step
     for (int k = 0; k + 1 < ncp; ++k) {
         rc = save_checkpoint(c, k);
-        if (!rc) rc = run_steps(c, c->seg);
+        const long long plain = std::max(0LL, c->seg - 2);
+        if (!rc) rc = run_steps(c, plain);
+        for (long long s = plain; s < c->seg && !rc; ++s) {
+            rc = launch_step(c, nullptr, nullptr, 0, true, nullptr, true);
+            if (!rc) ++c->steps;
+        }
         if (rc) return rc;
     }
     rc = save_checkpoint(c, ncp - 1);
@@ -1852,7 +1887,13 @@ static int reverse_segment(virs_ctx *c, int k, int &ap, int &aq, int &ar)
         p.nsoft = 0;
         HIPC(c, launch_dense(c, p, true));
         TailParams t = tail_params(c, c->af[ap], c->af[aq], c->af[ar], c->ctl_adj, ap, ar);
-        t.bs = boundary_state(c, c->ast);
+        t.bs = boundary_state(c);
+        // the reverse rings: dbar and ybar (FP64)
+        t.bs.ws[0] = t.bs.ws[1] = nullptr;
+        t.bs.xs[0] = c->ast[0];
+        t.bs.xs[1] = c->ast[1];
+        t.bs.ys[0] = c->ast[2];
+        t.bs.ys[1] = c->ast[3];
         t.bs.trace_d = c->trace_d[buf];
         t.bs.trace_y = c->trace_y[buf];
         t.bs.trace_base = s0;
