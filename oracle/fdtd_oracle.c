@@ -25,7 +25,8 @@
  *
  * Pinning (tests/test_oracle.py): rigid-box discrete eigenmodes, discrete
  * energy conservation, source/receiver reciprocity, scipy.signal.lfilter for
- * the DF1 recursion, a single-cell hand derivation of the boundary correction,
+ * the filter recursion (float direct form I and double direct form II), a
+ * single-cell hand derivation of the boundary correction,
  * central finite differences for the adjoint, brute-force numpy steps on tiny
  * grids for the float32 arithmetic order, and a literal-formula check of the
  * RMS stop on synthetic IRs.
@@ -398,8 +399,9 @@ int oracle_adjoint_f64(const oracle_desc *d, int64_t nsteps, const double *dL_di
 }
 
 /* ------------------------------------------------------------------------
- * 16-bit storage runner (FP16 or BF16 fields, FP32 arithmetic, FP64 filter
- * state), restating the kernels' scaled-storage rules exactly:
+ * 16-bit storage runner (FP16 or BF16 fields, FP32 arithmetic, the boundary
+ * filter in double as transposed direct form II with its two states stored
+ * in float), restating the kernels' scaled-storage rules exactly:
  *   - each stored field has a power-of-two exponent e (stored = value*2^e);
  *   - the exponent of step n's field is 14 - k (clamped to [-60, 60]),
  *     est = m*2^k, where
@@ -635,12 +637,16 @@ int oracle_run_s16(const oracle_desc *d, int bf16, int sr, int64_t nsteps, doubl
             const int m = d->bmat ? d->bmat[e] : 0;
             const double *c = d->coeffs + 5 * m;
             const float dif = ld16(&S, P[bi]) * sP - ld16(&S, Q[bi]) * sQ;
+            /* transposed direct form II in double, states s1, s2 stored as
+             * float: y = b0 d + s1, s1' = b1 d - a1 y + s2, s2' = b2 d - a2 y;
+             * this step's correction uses the double y */
             double *s4 = st + 4 * e;
             const double in = (double)dif;
-            const double yv = c[0] * in + c[1] * s4[0] + c[2] * s4[1] - c[3] * s4[2] - c[4] * s4[3];
-            /* the stored output is rounded to float; this step's correction
-             * uses the double value */
-            s4[1] = s4[0]; s4[0] = in; s4[3] = s4[2]; s4[2] = (double)(float)yv;
+            const double yv = c[0] * in + s4[0];
+            const double s1 = c[1] * in - c[3] * yv + s4[1];
+            const double s2 = c[2] * in - c[4] * yv;
+            s4[0] = (double)(float)s1;
+            s4[1] = (double)(float)s2;
             float r = ld16(&S, R[bi]) * sRi;
             r = (float)((double)r - (double)d->kconst * (yv * 0.1 + in * 0.9) * (double)0.4f);
             R[bi] = st16(&S, r * sR, (uint64_t)bi, n, SALT_BOUNDARY);

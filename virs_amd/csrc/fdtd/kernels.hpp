@@ -441,20 +441,23 @@ __device__ inline int edge_dist(int x, int y, int z, int nx, int ny, int nz)
 struct BoundaryState {
     long long nb;
     const uint8_t *bmat;       // material row per entry (zeros for one material)
-    // Filter state as two-slot rings indexed by step parity: on entry to
-    // step n, slot n&1 holds the value of step n-2 and slot (n+1)&1 that of
-    // step n-1; step n overwrites slot n&1, so a step writes one value per
-    // ring instead of shifting the state.
-    // FP32/FP64 fields: the direct-form-II state w in FP64 (ws), 8 bytes
-    // read twice and written once per entry and step.  xs/ys then hold the
+    // Filter state.  Rings are indexed by step parity: on entry to step n,
+    // slot n&1 holds the value of step n-2 and slot (n+1)&1 that of step
+    // n-1; step n overwrites slot n&1, so a step writes one value per ring
+    // instead of shifting the state.
+    // FP32/FP64 fields: ws = the direct-form-II state w as an FP64 ring, 8
+    // bytes read twice and written once per entry and step.  xs/ys = the
     // history of the filter inputs d (in the compute type C, which holds d
-    // exactly: a difference of two C values) and outputs y (FP64): written
-    // only when non-null (the recording forward's last two steps before
-    // each snapshot, which seed the adjoint's trace), never read by a step.
-    // 16-bit fields: direct form I, xs = inputs d in FP32 and ys = outputs
-    // y stored in FP32 (YsOf), read and written every step; ws unused.
+    // exactly: a difference of two C values) and outputs y (FP64), rings
+    // written only when non-null (the recording forward's last two steps
+    // before each snapshot, which seed the adjoint's trace) and never read
+    // by a step.
+    // 16-bit fields: ts = the transposed-direct-form-II states s1, s2,
+    // computed in FP64 and stored in FP32, read and rewritten in place every
+    // step (16 bytes per entry and step).
     // Reverse: xs = dbar, ys = ybar, both FP64.
     void *ws[2];
+    void *ts[2];
     void *xs[2];
     void *ys[2];
     const double *coeffs;      // [nmat][5] b0 b1 b2 a1 a2
@@ -498,39 +501,22 @@ struct DenseParams {
     BoundaryState bs;
 };
 
-// Element type of the filter-output ring for field storage T: FP64, or
-// FP32 for the 16-bit formats, whose storage rounding is far coarser than
-// FP32's (the reference keeps all of its filter state in float).  The
-// output is still computed in double; only the stored copy is rounded.
-template <typename T>
-struct YsOf {
-    using type = double;
-};
-template <>
-struct YsOf<f16> {
-    using type = float;
-};
-template <>
-struct YsOf<bf16> {
-    using type = float;
-};
-
 // Biquad step and boundary correction for one list entry (simulator_gpu.cu:126-148).
 // The reference's filter is y = b0 d + b1 d^{n-1} + b2 d^{n-2} - a1 y^{n-1} - a2 y^{n-2}
-// with float state.  The same filter is run in double in one of two forms:
-//   direct form I (16-bit fields), x1 = d^{n-1}, x2 = d^{n-2}, y1 = y^{n-1}, y2 = y^{n-2}:
-//     y = b0 d + b1 x1 + b2 x2 - a1 y1 - a2 y2
-//   direct form II (FP32/FP64 fields), w1 = w^{n-1}, w2 = w^{n-2}:
+// (direct form I) with float state.  The same filter runs in double here, in
+// a form that keeps two values of state per entry instead of four:
+//   direct form II (FP32/FP64 fields), w1 = w^{n-1}, w2 = w^{n-2} in FP64:
 //     w = d - a1 w1 - a2 w2,   y = b0 w + b1 w1 + b2 w2
-// Both give the same y in exact arithmetic; form II keeps two FP64 values
-// of state per entry instead of two inputs and two outputs.  Then
+//   transposed direct form II (16-bit fields), states s1, s2 stored in FP32:
+//     y = b0 d + s1,   s1' = b1 d - a1 y + s2,   s2' = b2 d - a2 y
+// Every form gives the same y in exact arithmetic.  The transposed form's
+// states are of the size of y, so their FP32 rounding (~6e-8 relative) stays
+// far below the 16-bit fields' storage noise; form II's w can exceed d by
+// the filter's low-frequency gain (1e4 for the reference's default
+// high-pass row), so it is kept in FP64.  Then
 //   R = R - K (0.1 y + 0.9 d) 0.4f             (evaluated in double)
 // The arithmetic shared by the list launch and the stencil's boundary phase
 // (one expression order for both).
-__device__ inline double biquad_out(const double *cf, double in, double x1, double x2, double y1, double y2)
-{
-    return cf[0] * in + cf[1] * x1 + cf[2] * x2 - cf[3] * y1 - cf[4] * y2;
-}
 __device__ inline double df2_state(const double *cf, double in, double w1, double w2)
 {
     return in - cf[3] * w1 - cf[4] * w2;
@@ -539,28 +525,34 @@ __device__ inline double df2_out(const double *cf, double w, double w1, double w
 {
     return cf[0] * w + cf[1] * w1 + cf[2] * w2;
 }
+__device__ inline double tdf2_out(const double *cf, double in, double s1) { return cf[0] * in + s1; }
+__device__ inline double tdf2_s1(const double *cf, double in, double y, double s2)
+{
+    return cf[1] * in - cf[3] * y + s2;
+}
+__device__ inline double tdf2_s2(const double *cf, double in, double y) { return cf[2] * in - cf[4] * y; }
 template <typename C>
 __device__ inline C corrected(C r, float kconst, double yv, double in)
 {
     return (C)((double)r - (double)kconst * (yv * 0.1 + in * 0.9) * (double)0.4f);
 }
 
-// Stores of entry e at step n: the new filter state (FP64 w for FP32/FP64
-// fields, with the d/y history when its rings are given; d and y for 16-bit
-// fields) and, when recording, the trace of the adjoint's segment.
-template <typename C, typename Y, bool DF2>
-__device__ inline void store_taps(const BoundaryState &bs, long long e, long long n, C dif, double yv,
-                                  double wv)
+// Stores of entry e at step n with 16-bit fields: the new states.
+__device__ inline void store_tdf2(const BoundaryState &bs, long long e, double s1, double s2)
 {
-    if constexpr (DF2) {
-        ((double *)bs.ws[n & 1])[e] = wv;
-        if (bs.xs[0]) {
-            ((C *)bs.xs[n & 1])[e] = dif;
-            ((Y *)bs.ys[n & 1])[e] = (Y)yv;
-        }
-    } else {
+    ((float *)bs.ts[0])[e] = (float)s1;
+    ((float *)bs.ts[1])[e] = (float)s2;
+}
+// Stores of entry e at step n with FP32/FP64 fields: the new state w, the
+// d/y history when its rings are given and, when recording, the trace of
+// the adjoint's segment.
+template <typename C>
+__device__ inline void store_df2(const BoundaryState &bs, long long e, long long n, C dif, double yv, double wv)
+{
+    ((double *)bs.ws[n & 1])[e] = wv;
+    if (bs.xs[0]) {
         ((C *)bs.xs[n & 1])[e] = dif;
-        ((Y *)bs.ys[n & 1])[e] = (Y)yv;
+        ((double *)bs.ys[n & 1])[e] = yv;
     }
     if (bs.trace_d) {
         const long long slot = n - bs.trace_base + 2;
@@ -690,7 +682,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
 
     // cell codes of one plane: V bytes, or V nibbles in one word for 16-bit
     using CD = typename std::conditional<S::kScaled, uint32_t, VB>::type;
-    using Y = typename YsOf<T>::type;
     struct Stage {
         VT pp, q, yn, ys; // P(z+1), Q(z), P(z,y+1), P(z,y-1) as stored
         CD cd;            // cell codes of plane z
@@ -700,9 +691,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
         // filter-input and filter-output rings (lanes past the count hold
         // the slice's last entry)
         uint32_t efirst, ecount, eb, em;
-        C ex0, ex1;  // direct form I (16-bit fields)
-        Y ey0, ey1;
-        double ew0, ew1; // direct form II (FP32/FP64 fields)
+        double ew0, ew1; // direct form II ring (FP32/FP64 fields)
+        float es1, es2;  // transposed direct form II states (16-bit fields)
     };
     // FUSED: the cells' (d, stored p^{n+1}) of one wave and plane, by
     // position j in the lane's vector and lane, so an entry's lane can read
@@ -799,10 +789,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                 st.eb = p.bidx[ec];
                 st.em = p.bs.bmat[ec];
                 if constexpr (S::kScaled) {
-                    st.ex0 = ((const C *)p.bs.xs[0])[ec];
-                    st.ex1 = ((const C *)p.bs.xs[1])[ec];
-                    st.ey0 = ((const Y *)p.bs.ys[0])[ec];
-                    st.ey1 = ((const Y *)p.bs.ys[1])[ec];
+                    st.es1 = ((const float *)p.bs.ts[0])[ec];
+                    st.es2 = ((const float *)p.bs.ts[1])[ec];
                 } else {
                     st.ew0 = ((const double *)p.bs.ws[0])[ec];
                     st.ew1 = ((const double *)p.bs.ws[1])[ec];
@@ -1025,19 +1013,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                     const bool odd = (n & 1) != 0;
                     for (uint32_t r0 = 0; r0 < cnt; r0 += 64) {
                         uint32_t b = cur.eb, m = cur.em;
-                        C xa = cur.ex0, xb = cur.ex1;
-                        Y ya = cur.ey0, yb = cur.ey1;
                         double wa = cur.ew0, wb = cur.ew1;
+                        float sa = cur.es1, sb = cur.es2;
                         const long long e = (long long)cur.efirst + r0 + lane;
                         if (r0 > 0) { // entries past the first 64 (floor and ceiling planes)
                             const long long ec = min(e, (long long)cur.efirst + cnt - 1);
                             b = p.bidx[ec];
                             m = p.bs.bmat[ec];
                             if constexpr (S::kScaled) {
-                                xa = ((const C *)p.bs.xs[0])[ec];
-                                xb = ((const C *)p.bs.xs[1])[ec];
-                                ya = ((const Y *)p.bs.ys[0])[ec];
-                                yb = ((const Y *)p.bs.ys[1])[ec];
+                                sa = ((const float *)p.bs.ts[0])[ec];
+                                sb = ((const float *)p.bs.ts[1])[ec];
                             } else {
                                 wa = ((const double *)p.bs.ws[0])[ec];
                                 wb = ((const double *)p.bs.ws[1])[ec];
@@ -1049,11 +1034,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                             const C dif = sl.d;
                             const double in = (double)dif;
                             const double *cf = cf_lds + 5 * m;
-                            double yv, wv = 0.0;
+                            double yv, wv = 0.0, s1 = 0.0, s2 = 0.0;
                             if constexpr (S::kScaled) {
-                                const double x1 = (double)(odd ? xa : xb), x2 = (double)(odd ? xb : xa);
-                                const double y1 = (double)(odd ? ya : yb), y2 = (double)(odd ? yb : ya);
-                                yv = biquad_out(cf, in, x1, x2, y1, y2);
+                                yv = tdf2_out(cf, in, (double)sa);
+                                s1 = tdf2_s1(cf, in, yv, (double)sb);
+                                s2 = tdf2_s2(cf, in, yv);
                             } else {
                                 const double w1 = odd ? wa : wb, w2 = odd ? wb : wa;
                                 wv = df2_state(cf, in, w1, w2);
@@ -1062,7 +1047,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                             const C r = corrected<C>(sl.r * kst_inv, p.bs.kconst, yv, in);
                             flag_nonfinite(p.ctl, r);
                             const T t = store_round<T>(r * kst, SR, (unsigned long long)b, n, kSaltBoundary);
-                            store_taps<C, Y, !S::kScaled>(p.bs, e, n, dif, yv, wv);
+                            if constexpr (S::kScaled) store_tdf2(p.bs, e, s1, s2);
+                            else store_df2<C>(p.bs, e, n, dif, yv, wv);
                             sl.r = S::ld(t);
                         }
                     }
@@ -1214,7 +1200,6 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
 {
     using S = Store<T>;
     using C = typename S::C;
-    using Y = typename YsOf<T>::type;
     const long long n = p.ctl->nring[p.slot];
     const bool skip = (n & kStopBit) != 0;
     C sP = (C)1, sQ = (C)1, sR = (C)1, sRi = (C)1;
@@ -1240,12 +1225,11 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
         uint32_t bv[U];
         uint8_t mv[U];
         T h0[U], h1[U], rt[U];
-        // filter state: direct form II (FP64 w) for FP32/FP64 fields, direct
-        // form I (FP32 inputs and outputs) for 16-bit fields
+        // filter state: direct form II (FP64 w ring) for FP32/FP64 fields,
+        // transposed direct form II (FP32 s1, s2) for 16-bit fields
         constexpr bool kDF2 = !S::kScaled;
-        C xa[U], xb[U];
-        Y ya[U], yb[U];
         double wa[U], wb[U];
+        float sa[U], sb[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             ev[u] = ((long long)(blockIdx.x - 1) * U + u) * kTailBlock + threadIdx.x;
@@ -1258,10 +1242,8 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
                 wa[u] = ((const double *)p.bs.ws[0])[ec];
                 wb[u] = ((const double *)p.bs.ws[1])[ec];
             } else {
-                xa[u] = ((const C *)p.bs.xs[0])[ec];
-                xb[u] = ((const C *)p.bs.xs[1])[ec];
-                ya[u] = ((const Y *)p.bs.ys[0])[ec];
-                yb[u] = ((const Y *)p.bs.ys[1])[ec];
+                sa[u] = ((const float *)p.bs.ts[0])[ec];
+                sb[u] = ((const float *)p.bs.ts[1])[ec];
             }
         }
 #pragma unroll
@@ -1279,21 +1261,22 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
             if constexpr (S::kScaled) pp = pp * sP, qq = qq * sQ, r = r * sRi;
             const C dif = pp - qq;
             const double in = (double)dif;
-            double yv, wv = 0.0;
+            double yv, wv = 0.0, s1 = 0.0, s2 = 0.0;
             if constexpr (kDF2) {
                 const double w1 = odd ? wa[u] : wb[u], w2 = odd ? wb[u] : wa[u];
                 wv = df2_state(cf, in, w1, w2);
                 yv = df2_out(cf, wv, w1, w2);
             } else {
-                const double x1 = (double)(odd ? xa[u] : xb[u]), x2 = (double)(odd ? xb[u] : xa[u]);
-                const double y1 = (double)(odd ? ya[u] : yb[u]), y2 = (double)(odd ? yb[u] : ya[u]);
-                yv = biquad_out(cf, in, x1, x2, y1, y2);
+                yv = tdf2_out(cf, in, (double)sa[u]);
+                s1 = tdf2_s1(cf, in, yv, (double)sb[u]);
+                s2 = tdf2_s2(cf, in, yv);
             }
             r = corrected<C>(r, p.bs.kconst, yv, in);
             T t = store_round<T>(r * sR, p.sr, (unsigned long long)b, n, kSaltBoundary);
             if (e < nb && !skip) {
                 flag_nonfinite(p.ctl, r);
-                store_taps<C, Y, kDF2>(p.bs, e, n, dif, yv, wv);
+                if constexpr (kDF2) store_df2<C>(p.bs, e, n, dif, yv, wv);
+                else store_tdf2(p.bs, e, s1, s2);
                 // a soft source on this cell is added to the stored corrected
                 // value (the point tables are scanned only when some point is
                 // on the list)

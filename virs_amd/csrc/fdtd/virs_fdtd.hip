@@ -61,11 +61,12 @@ struct virs_ctx {
     long long nb = 0;
     uint32_t *bidx = nullptr;
     uint8_t *bmat = nullptr;
-    // filter state rings (BoundaryState): st[0..1] filter inputs d in the
-    // compute type (xs), st[2..3] outputs y (ys: FP64, FP32 for 16-bit
-    // fields), st[4..5] the direct-form-II state w in FP64 (ws, FP32/FP64
-    // fields only; there xs/ys are the d/y history the adjoint's recording
-    // writes before each snapshot)
+    // filter state (BoundaryState).  FP32/FP64 fields: st[0..1] the history
+    // of the filter inputs d in the compute type (xs), st[2..3] of the
+    // outputs y in FP64 (ys), which the adjoint's recording writes before
+    // each snapshot, and st[4..5] the direct-form-II state ring w in FP64
+    // (ws).  16-bit fields: st[0..1] the transposed-direct-form-II states
+    // s1, s2 in FP32 (ts).
     void *st[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     void *hv[2] = {nullptr, nullptr}; // list-cell pressures by step parity (TailParams::hv)
     std::vector<uint32_t> h_bidx;
@@ -282,23 +283,26 @@ int upload_points(virs_ctx *c)
 
 bool is16(const virs_ctx *c);
 
-// The forward filter state of a step: direct form I rings for 16-bit
-// fields; the direct-form-II rings for FP32/FP64 fields, with the d/y
-// history rings only when `history` is set (BoundaryState).
+// The forward filter state of a step: the transposed-direct-form-II states
+// for 16-bit fields; the direct-form-II ring for FP32/FP64 fields, with the
+// d/y history rings only when `history` is set (BoundaryState).
 BoundaryState boundary_state(const virs_ctx *c, bool history = false)
 {
     BoundaryState b{};
     b.nb = c->nb;
     b.bmat = c->bmat;
-    if (is16(c) || history) {
-        b.xs[0] = c->st[0];
-        b.xs[1] = c->st[1];
-        b.ys[0] = c->st[2];
-        b.ys[1] = c->st[3];
-    }
-    if (!is16(c)) {
+    if (is16(c)) {
+        b.ts[0] = c->st[0];
+        b.ts[1] = c->st[1];
+    } else {
         b.ws[0] = c->st[4];
         b.ws[1] = c->st[5];
+        if (history) {
+            b.xs[0] = c->st[0];
+            b.xs[1] = c->st[1];
+            b.ys[0] = c->st[2];
+            b.ys[1] = c->st[3];
+        }
     }
     b.coeffs = c->coeffs;
     b.nmat = c->nmat;
@@ -643,17 +647,18 @@ bool is16(const virs_ctx *c) { return c->d.precision == VIRS_FP16 || c->d.precis
 // cell codes: a byte per cell, half a byte in the 16-bit formats
 size_t code_bytes(const virs_ctx *c) { return is16(c) ? (size_t)(c->N + 1) / 2 : (size_t)c->N; }
 
-// bytes of filter-state ring k (virs_ctx::st): 0 and 1 hold inputs in the
-// compute type, 2 and 3 outputs in FP64 (FP32 for 16-bit storage, YsOf), 4
-// and 5 the direct-form-II state in FP64 (none for 16-bit storage)
+// bytes of filter-state array k (virs_ctx::st): FP32/FP64 fields: 0 and 1
+// hold inputs in the compute type, 2 and 3 outputs in FP64, 4 and 5 the
+// direct-form-II state in FP64; 16-bit fields: 0 and 1 the states s1, s2
+// in FP32
 size_t state_bytes(const virs_ctx *c, int k)
 {
-    const size_t ys = is16(c) ? sizeof(float) : sizeof(double);
-    const size_t el = k < 2 ? compute_bytes(c) : k < 4 ? ys : is16(c) ? 0 : sizeof(double);
+    const size_t el = is16(c) ? (k < 2 ? sizeof(float) : 0)
+                              : (k < 2 ? compute_bytes(c) : sizeof(double));
     return el * (size_t)std::max<long long>(1, c->nb);
 }
-// filter-state rings in use: six with FP32/FP64 fields, four with 16-bit
-int state_rings(const virs_ctx *c) { return is16(c) ? 4 : 6; }
+// filter-state arrays in use: six with FP32/FP64 fields, two with 16-bit
+int state_rings(const virs_ctx *c) { return is16(c) ? 2 : 6; }
 // bytes of all filter-state rings (one snapshot's share)
 size_t state_total(const virs_ctx *c)
 {
