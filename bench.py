@@ -75,13 +75,12 @@ def dense_bytes_of(n, s):
 
 
 def list_entry_bytes(s):
-    """Algorithmic bytes per list entry: 4 index + 1 material + 2c+c
-    filter-input ring (c = compute type: 4, or 8 for FP64 fields) + 2y+y
-    filter-output ring (y = 8, or 4 for 16-bit fields) + 2s+s pressure ring
-    + s+s p^{n+1} read and written."""
-    c = 8 if s == 8 else 4
-    y = 4 if s == 2 else 8
-    return 5 + 3 * c + 3 * y + 5 * s
+    """Algorithmic bytes per list entry: 4 index + 1 material + the filter
+    state (FP32/FP64 fields: the direct-form-II ring, two FP64 values read
+    and one written, 24; 16-bit fields: two FP32 states read and written,
+    16) + 2s+s pressure ring + s+s p^{n+1} read and written."""
+    state = 16 if s == 2 else 24
+    return 5 + state + 5 * s
 
 
 def step_bytes_fp64_state(n, nb, nrcv, s):
