@@ -10,11 +10,12 @@
 - checkpoints beyond the device memory fail with an error code, and the
   context stays usable;
 - the RMS stop, held off while timed, recorded and adjoint steps run, is in
-  effect again afterwards."""
+  effect again afterwards;
+- contexts on one GPU share no state."""
 import numpy as np
 import pytest
 
-from helpers import gpu_context, tiny_problem
+from helpers import gpu_context, oracle_problem, tiny_problem
 from oracle import oracle as O
 from virs_amd import grid as G
 from virs_amd.fdtd import FDTD, FDTDError
@@ -228,3 +229,32 @@ def test_rms_stop_is_restored_after_timing_recording_and_adjoint(gpu):
     assert np.isfinite(f.adjoint(np.ones((1, steps), np.float32))).all()
     stops_at_oracle_step("an adjoint")
     f.close()
+
+
+def test_contexts_interleaved_on_one_gpu(gpu):
+    """Three contexts on one GPU (different rooms and storage precisions, with
+    and without graphs), stepped in alternation with uneven batches, each
+    give the bits of their own run alone: no state is shared between
+    contexts.  The reference keeps its constants in process-global
+    __constant__ memory uploaded once per process (simulator_gpu.cu:21-36,
+    189-212), so a second simulator in one process would reuse the first
+    one's grid dimensions."""
+    pa = tiny_problem(41, dims=(24, 18, 14), steps=70)
+    pb = tiny_problem(42, dims=(20, 16, 13), steps=70, nmat=3)
+    pc = tiny_problem(43, dims=(32, 12, 10), steps=70, hard=False)
+    _say("three contexts interleaved: fp32 graphs, fp64 direct, fp16 graphs")
+    fa = gpu_context(pa, "fp32", use_graph=True)
+    fb = gpu_context(pb, "fp64")
+    fc = gpu_context(pc, "fp16", use_graph=True)
+    for n in (7, 30, 1, 32):
+        for f in (fa, fb, fc):
+            f.step(n)
+    for f in (fa, fb, fc):
+        f.synchronize()
+    ra = oracle_problem(pa).run(70, "f32", want_fields=True)
+    rb = oracle_problem(pb).run(70, "f64", want_fields=True)
+    rc = oracle_problem(pc).run16(70, "fp16", stochastic=True, want_fields=True)
+    for f, r in ((fa, ra), (fb, rb), (fc, rc)):
+        np.testing.assert_array_equal(f.receivers(f64=True), r["ir"])
+        np.testing.assert_array_equal(f.field(0), r["p"])
+        f.close()

@@ -209,3 +209,48 @@ def test_chunks_ending_mid_row(gpu, prec, nx):
     np.testing.assert_array_equal(outs[0][1], ref["p"])
     np.testing.assert_array_equal(outs[1][0], outs[0][0])
     np.testing.assert_array_equal(outs[1][1], outs[0][1])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_fields_written_mid_run_keep_the_filter_state(gpu, precision):
+    """Fields written between steps (virs_write_field_f64, the reference's
+    updateGPUFromGrid, simulator_dispatch.cpp:70-81) replace p^n and p^{n-1}
+    and leave the boundary filter's state as it was, as the reference's
+    device filter_states are left.  Writing the fields back unchanged gives
+    the bits of an uninterrupted run; writing them halved gives the bits of
+    the oracle continued from the halved fields with its filter state
+    carried over (state_io)."""
+    p = tiny_problem(44, steps=70)
+    n1, n = 35, p["steps"]
+    prec = "f32" if precision == "fp32" else "f64"
+    _say(f"mid-run field write {precision}")
+    straight = gpu_context(p, precision)
+    straight.step(n)
+    straight.synchronize()
+    same = gpu_context(p, precision, use_graph=True)
+    same.step(n1)
+    P, Q = same.field(0), same.field(1)
+    same.write_field(0, P)
+    same.write_field(1, Q)
+    same.step(n - n1)
+    same.synchronize()
+    np.testing.assert_array_equal(same.receivers(f64=True), straight.receivers(f64=True))
+    np.testing.assert_array_equal(same.field(0), straight.field(0))
+    half = gpu_context(p, precision)
+    half.step(n1)
+    half.write_field(0, 0.5 * half.field(0))
+    half.write_field(1, 0.5 * half.field(1))
+    half.step(n - n1)
+    half.synchronize()
+    o = oracle_problem(p)
+    o.state_io = np.zeros(4 * max(1, len(p["bidx"])), np.float64)
+    first = o.run(n1, prec, want_fields=True)
+    o.init_p = np.ascontiguousarray(0.5 * first["p"].reshape(-1))
+    o.init_q = np.ascontiguousarray(0.5 * first["q"].reshape(-1))
+    second = o.run(n - n1, prec, n0=n1, want_fields=True)
+    got = half.receivers(f64=True)
+    np.testing.assert_array_equal(got[:, :n1], first["ir"])
+    np.testing.assert_array_equal(got[:, n1:], second["ir"])
+    np.testing.assert_array_equal(half.field(0), second["p"])
+    for f in (straight, same, half):
+        f.close()
