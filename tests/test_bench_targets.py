@@ -44,3 +44,18 @@ def test_launch_label_says_what_ran():
             == "16 replays of a 30-step HIP graph + one 20-step HIP graph")
     assert b.launch_label(dict(base, graph_launches=3, graph_steps=30)) == "3 replays of a 30-step HIP graph"
     assert b.launch_label(dict(base, single_steps=20)) == "20 single-launch steps"
+
+
+def test_more_than_one_gpu_is_refused():
+    """The path runs on one GPU: --gpus N > 1 (or a WORLD_SIZE above 1)
+    exits with status 2 and says so, before any GPU work."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    env = dict(os.environ, WORLD_SIZE="2")
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py")], capture_output=True, text=True,
+                       timeout=120, env=env)
+    assert r.returncode == 2
