@@ -194,3 +194,39 @@ def test_write_and_read_field_round_trip(gpu):
         if prec in ("fp16", "bf16"):
             assert np.all(got[zero] == 0.0)
         f.close()
+
+
+@pytest.mark.parametrize("fmt", ["fp16", "bf16"])
+@pytest.mark.parametrize("k", [-40, 30])
+def test_16bit_inputs_scaled_by_a_power_of_two_shift_only_the_exponents(gpu, fmt, k):
+    """Every step of the 16-bit scheme commutes with scaling by 2^k: the
+    stored bits stay the same and each field's exponent moves by -k, so the
+    IR and the fields scale exactly.  Both sources are scaled: the soft
+    signals and the hard cell's pulse (10 for n < 10, given as a signal).
+    The scaled run is also bit-exact against the oracle's run of the scaled
+    room, at exponents near the +-60 clamp's far ends (about 50 for 2^-40)."""
+    p = tiny_problem(46, steps=60, hard=True)
+    n = p["steps"]
+    pulse = np.where(np.arange(n) < 10, 10.0, 0.0)
+    q = dict(p)
+    q["soft_sig"] = p["soft_sig"] * 2.0 ** k
+    print(f"16-bit {fmt} sources scaled by 2^{k}", flush=True)
+    runs = []
+    for prob, hs in ((p, pulse), (q, pulse * 2.0 ** k)):
+        f = gpu_context(prob, fmt)
+        f.set_hard_source(hs)
+        f.start(n)
+        f.step(n)
+        f.synchronize()
+        runs.append((f.receivers(True), f.field(0), f.field_exponent(0)))
+        f.close()
+    (ir0, p0, e0), (ir1, p1, e1) = runs
+    np.testing.assert_array_equal(ir1, ir0 * 2.0 ** k)
+    np.testing.assert_array_equal(p1, p0 * 2.0 ** k)
+    assert e1 == e0 - k
+    pb = oracle_problem(q)
+    pb.hard_sig = np.ascontiguousarray(pulse * 2.0 ** k)
+    ref = pb.run16(n, fmt, True, want_fields=True)
+    np.testing.assert_array_equal(ir1, ref["ir"])
+    np.testing.assert_array_equal(p1, ref["p"])
+    assert ref["exp"].max() <= 60 and ref["exp"].min() >= -60
