@@ -73,6 +73,34 @@ def test_random_room_fp32_first_steps_bit_exact(gpu, random_room):
     np.testing.assert_array_equal(q, ref["q"])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp16"])
+def test_random_room_first_steps_bit_exact_in_other_precisions(gpu, random_room, precision):
+    """FP64 storage (the oracle-grade mode) and FP16 storage over the first
+    100 steps of the full 512x384x256 room, bit for bit against the oracle's
+    FP64 run and its 16-bit runner (stochastic rounding, the same hash)."""
+    from oracle import oracle as O
+    from virs_amd.fdtd import setup
+    steps = 100
+    room, sc = random_room.room, random_room.scenario
+    _say(f"random room {precision} vs the oracle, {steps} steps")
+    f = setup(random_room, precision, use_graph=True)
+    f.start(steps)
+    f.step(steps)
+    ir = f.receivers(True)
+    p = f.field(0)
+    f.close()
+    O.set_threads(os.cpu_count() or 1)
+    pb = O.Problem(room.flags, sc.h, dt=random_room.dt, bidx=room.bidx, bmat=room.bmat, coeffs=sc.coeffs,
+                   soft_idx=random_room.src_idx, soft_sig=random_room.src_sig, rcv_idx=random_room.rcv_idx)
+    if precision == "fp64":
+        ref = pb.run(steps, "f64", want_fields=True)
+    else:
+        ref = pb.run16(steps, "fp16", True, want_fields=True)
+    assert np.abs(ref["p"]).max() > 0
+    np.testing.assert_array_equal(ir, ref["ir"])
+    np.testing.assert_array_equal(p, ref["p"])
+
+
 def _coords(i, dims):
     nx, ny, _ = dims
     return np.array([i % nx, (i // nx) % ny, i // (nx * ny)])
