@@ -35,9 +35,10 @@ extern "C" {
 
 /* Storage precision of the pressure fields.  FP16/BF16 store p^n and p^{n-1}
  * with a per-field power-of-two scale and compute in FP32; FP64 is the
- * reference-scheme oracle.  Boundary-filter outputs are kept in FP64 with
- * FP32/FP64 fields and in FP32 with 16-bit fields (the reference keeps all
- * of its filter state in float); they are computed in double in every mode. */
+ * reference-scheme oracle.  The boundary filter is computed in double in
+ * every mode, with two values of state per list entry: kept in FP64 with
+ * FP32/FP64 fields, stored in FP32 with 16-bit fields (the reference keeps
+ * four values of filter state per entry, in float). */
 #define VIRS_FP32 0
 #define VIRS_FP16 1
 #define VIRS_BF16 2
