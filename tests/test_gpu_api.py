@@ -159,7 +159,7 @@ def test_timed_steps_are_virs_steps(gpu, n):
 
 
 def test_checkpoints_beyond_device_memory_fail_with_an_error(gpu):
-    """SURVEY's allocation-failure case forced from the host: 4000 FP64
+    """An allocation failure forced from the host: 4000 FP64
     checkpoints of a 256^3 room (2 x 134 MB each, ~1.1 TB) fail at once with
     VIRS_ERR_NOMEM, without touching the device's memory, and the context
     records and differentiates normally afterwards."""
