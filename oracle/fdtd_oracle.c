@@ -12,7 +12,7 @@
  *     hard source       src/simulator/simulator_gpu.cu:59-63,214-218
  *     rotate + receiver src/simulator/simulator_gpu.cu:292-300
  *     constants         src/simulator/simulator_gpu.cu:195-198,147
- *   Extensions of this build (no reference counterpart), each reducing to the
+ *   Extensions of the GPU path (no reference counterpart), each reducing to the
  *   reference when switched off: per-voxel material rows, FP64 filter state,
  *   soft point sources (added after the boundary correction), several
  *   receivers, graded sponge damping.
