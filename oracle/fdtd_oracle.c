@@ -13,8 +13,10 @@
  *     rotate + receiver src/simulator/simulator_gpu.cu:292-300
  *     constants         src/simulator/simulator_gpu.cu:195-198,147
  *   Extensions of the GPU path (no reference counterpart), each reducing to the
- *   reference when switched off: per-voxel material rows, FP64 filter state,
- *   soft point sources (added after the boundary correction), several
+ *   reference when switched off: per-voxel material rows, the filter in
+ *   double with two values of state (direct form II, FP64; transposed
+ *   direct form II with float-stored states in the 16-bit runner), soft
+ *   point sources (added after the boundary correction), several
  *   receivers, graded sponge damping.
  *   oracle_rms_continue   src/simulator/simulator_gpu.cu:303-330
  *   oracle_refcpu_steps   src/simulator/simulator_cpu.cpp:55-81 with
