@@ -102,7 +102,9 @@ int virs_upload_normals(virs_ctx *ctx, const uint8_t *normals);
  * Filter state is zeroed. */
 int virs_upload_boundary(virs_ctx *ctx, const uint32_t *idx, const uint8_t *mat, int64_t nb);
 /* allocFilterCoeffs + uploadFilterCoeffsToGPU (simulator_dispatch.cpp:162-280):
- * coeffs[nmat][5] = b0, b1, b2, a1, a2 (a0 = 1). */
+ * coeffs[nmat][5] = b0, b1, b2, a1, a2 (a0 = 1).  The filter state is kept:
+ * set between steps, the new rows continue from the stored two-value state
+ * (the reference sets its rows once, before its time loop). */
 int virs_set_materials(virs_ctx *ctx, const double *coeffs, int32_t nmat);
 
 /* Flag-3 (hard) source value per step.  sig == NULL restores the reference
