@@ -24,6 +24,13 @@
  *   oracle_adjoint_f64    reverse-mode transpose of oracle_run_f64 (no reference
  *                         counterpart), written as a literal scatter-form
  *                         transpose of the forward code.
+ *   oracle_run_s16        the step with FP16/BF16 field storage and one
+ *                         power-of-two scale per field (no reference
+ *                         counterpart): the GPU path's 16-bit scheme, which
+ *                         restates the dense update above in exact arithmetic
+ *                         with walls held at zero.
+ *   oracle_round16        its storage rounding (nearest, or stochastic from the
+ *                         cell/step hash), exposed for the tests.
  *
  * Pinning (tests/test_oracle.py): rigid-box discrete eigenmodes, discrete
  * energy conservation, source/receiver reciprocity, scipy.signal.lfilter for
@@ -31,7 +38,12 @@
  * single-cell hand derivation of the boundary correction,
  * central finite differences for the adjoint, brute-force numpy steps on tiny
  * grids for the float32 arithmetic order, and a literal-formula check of the
- * RMS stop on synthetic IRs.
+ * RMS stop on synthetic IRs.  The 16-bit runner: FP16 nearest rounding against
+ * numpy's float16 cast, every half pattern round-tripping, BF16 nearest as the
+ * nearest value by brute force, stochastic rounding exactly unbiased over every
+ * random-bit value, inputs scaled by 2^k giving the same stored bits with the
+ * exponents shifted by -k, the exactness in float of the products the fused
+ * multiply-adds rely on, and the runner tracking the FP64 scheme.
  *
  * Built with -ffp-contract=off so every multiply and add rounds separately.
  */
