@@ -211,6 +211,33 @@ def test_chunks_ending_mid_row(gpu, prec, nx):
     np.testing.assert_array_equal(outs[1][1], outs[0][1])
 
 
+@pytest.mark.parametrize("prec,nx", [("fp32", 384), ("fp32", 512), ("fp32", 516), ("fp16", 768),
+                                     ("fp16", 1024), ("bf16", 1032), ("fp64", 256), ("fp64", 258)])
+def test_rows_staged_through_lds_and_not(gpu, prec, nx):
+    """The stencil stages each plane through LDS when a row has at most 128
+    vectors (FP32 rows up to 512 cells, 16-bit up to 1024, FP64 up to 256)
+    and loads the rows +-Nx itself above that.  Both sides of the limit,
+    with chunks that end mid-row (Nx = 384 FP32, 768 FP16) and with several
+    chunks per block (a 7-block grid), give the oracle's results bit for
+    bit."""
+    p = tiny_problem(16, dims=(nx, 9, 10), steps=30)
+    _say(f"rows staged or not {prec} nx={nx}")
+    outs = []
+    for nblk in (0, 7):
+        f = gpu_context(p, prec, dense_blocks=nblk)
+        f.step(p["steps"])
+        outs.append((f.receivers(True), f.field(0)))
+        f.close()
+    pb = oracle_problem(p)
+    if prec in ("fp32", "fp64"):
+        ref = pb.run(p["steps"], "f32" if prec == "fp32" else "f64", want_fields=True)
+    else:
+        ref = pb.run16(p["steps"], prec, True, want_fields=True)
+    for ir, fld in outs:
+        np.testing.assert_array_equal(ir, ref["ir"])
+        np.testing.assert_array_equal(fld, ref["p"])
+
+
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_fields_written_mid_run_keep_the_filter_state(gpu, precision):
     """Fields written between steps (virs_write_field_f64, the reference's

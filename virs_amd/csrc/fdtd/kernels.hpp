@@ -21,6 +21,9 @@
 namespace virs {
 
 constexpr int kBlock = 256;   // 4 waves of 64
+// LDS staging of the stencil's planes (k_dense LY): rows of at most this
+// many vectors, so that a block's two halo rows take one vector per thread
+constexpr int kHaloVecs = kBlock / 2;
 constexpr int kWaves = kBlock / 64;
 constexpr int kSlots = 64;    // spread of the per-step |p| maximum atomics
 constexpr int kMaxSponge = 64;
@@ -569,10 +572,17 @@ __device__ inline void store_df2(const BoundaryState &bs, long long e, long long
 // A thread owns V consecutive x cells of a row of the flattened z-plane and
 // marches up in z: z neighbours stay in registers, x neighbours come from
 // the vector and the adjacent lanes, y neighbours are the rows +-Nx of the
-// same plane (L2 hits: in tile mode the neighbouring chunks march the same
-// z together).  The next plane's loads are issued before the current plane
+// same plane.  The next plane's loads are issued before the current plane
 // is computed.  Every cell's update is computed and the result selected by
 // cell type, without branches.
+// LY (rows of at most kHaloVecs vectors): each plane is staged in LDS.  The
+// block writes its chunk of P(z), which its threads hold anyway, and one
+// row on either side (a halo vector per thread, loaded with the plane), then
+// reads the rows +-Nx and the end lanes' x neighbours back after a barrier.
+// Two buffers alternate by plane, so one barrier per plane orders both the
+// writes before the reads and the reads before the next overwrite.
+// Otherwise each thread loads its rows +-Nx itself (L2 hits: in tile mode
+// the neighbouring chunks march the same z together).
 // 16-bit storage: arithmetic runs in p^n's stored units (value * 2^eP);
 // power-of-two scaling commutes exactly with every rounding step, so the
 // results equal a computation in physical units bit for bit.  Walls hold
@@ -584,7 +594,7 @@ __device__ inline void store_df2(const BoundaryState &bs, long long e, long long
 // HARD (16-bit storage): the room has flag-3 cells.  Without them a wall or
 // border cell (P = Q = 0 there) gets the update with a zero Laplacian
 // factor, which is exactly +0, in place of two selects per cell.
-template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE, bool SR, bool HARD = true>
+template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE, bool SR, bool HARD = true, bool LY = false>
 #ifndef VIRS_DENSE_WAVES
 #define VIRS_DENSE_WAVES 1 // minimum waves per SIMD the register allocation must allow
 #endif
@@ -684,6 +694,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
     using CD = typename std::conditional<S::kScaled, uint32_t, VB>::type;
     struct Stage {
         VT pp, q, yn, ys; // P(z+1), Q(z), P(z,y+1), P(z,y-1) as stored
+        VT h;             // LY: this thread's halo vector of P(z)
         CD cd;            // cell codes of plane z
         T tl, tr;         // P(z) left of lane 0 / right of lane 63
         // FUSED: the wave's list slice at plane z (first entry, count; wave
@@ -710,6 +721,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
         __syncthreads();
     }
 
+    // LY: two plane buffers of a chunk (slots 0..H-1 the row below it, H..
+    // its own vectors, then the row above); the buffer alternates by plane
+    static_assert(!(LY && FUSED), "the fused boundary keeps its own LDS use");
+    using RawV = typename Raw<sizeof(VT)>::type;
+    __shared__ RawV ybuf[LY ? 2 : 1][LY ? kHaloVecs + kBlock + kHaloVecs : 1];
+    const int H = nx / V; // vectors per row; LY: at most kHaloVecs
+    unsigned pcount = 0;  // planes staged so far (LY)
     for (long long w = wbeg; w < wend;) {
         const int pchunk = (int)(w / nz);
         const int z0 = (int)(w - (long long)pchunk * nz);
@@ -741,7 +759,31 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
         // vector-offset form; otherwise the extension is hoisted out of the
         // march and every access builds a 64-bit address.  Rewriting the
         // loop-carried copies themselves needs no register copies.
-        uint32_t xoc = oc, xon = on, xos = os, xob = S::kScaled ? ob >> 1 : ob;
+        // LY: thread t stages halo vector t of the chunk (t < 2H): vector
+        // v0 - H + t of the row below the chunk into slot t, or vector
+        // v0 + nv + t - H of the row above into slot nv + t; its own vector
+        // goes to slot H + t.  Halo vectors outside the plane are zero in
+        // both buffers for the whole chunk.
+        const long long v0 = (long long)pchunk * kBlock;
+        const int nv = (int)min((long long)kBlock, p.nvec_plane - v0);
+        const int tid = threadIdx.x;
+        const bool use_h = LY && tid < 2 * H;
+        const long long hvec = tid < H ? v0 - H + tid : v0 + nv + (tid - H);
+        const bool valid_h = use_h && hvec >= 0 && hvec < p.nvec_plane;
+        const int hslot = tid < H ? tid : nv + tid;
+        if constexpr (LY) {
+            __syncthreads(); // the previous chunk's last reads are done
+            if (use_h && !valid_h) {
+                RawV zr;
+                __builtin_memset(&zr, 0, sizeof(RawV));
+                ybuf[0][hslot] = zr;
+                ybuf[1][hslot] = zr;
+            }
+        }
+        // LY: the halo vector's offset in a plane takes xon's place (a valid
+        // vector when there is nothing to load)
+        const uint32_t oh = (uint32_t)((valid_h ? hvec : v0) * V) * (uint32_t)sizeof(T);
+        uint32_t xoc = oc, xon = LY ? oh : on, xos = os, xob = S::kScaled ? ob >> 1 : ob;
         // stochastic-rounding hash: cell0's part of the pair index times K1
         const uint32_t c0k1 = ((uint32_t)cell0 & ~1u) * kHashK1;
         // FUSED: this wave's row of the slice table and its first cell in a plane
@@ -769,10 +811,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
             } else {
                 st.cd = vload<VB, true>(reinterpret_cast<const VB *>(p.code + (long long)z * sxy + xob));
             }
-            st.yn = vload<VT, false>(reinterpret_cast<const VT *>(Pz + xon));
-            st.ys = vload<VT, false>(reinterpret_cast<const VT *>(Pz + xos));
-            if (need_l) st.tl = reinterpret_cast<const T *>(Pz + xc)[-1];
-            if (need_r) st.tr = reinterpret_cast<const T *>(Pz + xc)[V];
+            if constexpr (LY) {
+                st.h = vload<VT, false>(reinterpret_cast<const VT *>(Pz + xon));
+            } else {
+                st.yn = vload<VT, false>(reinterpret_cast<const VT *>(Pz + xon));
+                st.ys = vload<VT, false>(reinterpret_cast<const VT *>(Pz + xos));
+                if (need_l) st.tl = reinterpret_cast<const T *>(Pz + xc)[-1];
+                if (need_r) st.tr = reinterpret_cast<const T *>(Pz + xc)[V];
+            }
             if constexpr (FUSED) {
                 // every load goes out every plane, from an index clamped into
                 // the slice (one line when the slice is empty), so no load
@@ -830,7 +876,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
             // stored pairs (add_f16_half), without a conversion each
             constexpr bool kMixY = std::is_same<T, f16>::value && V >= 2;
             uint32_t ynw[kMixY ? V / 2 : 1], ysw[kMixY ? V / 2 : 1];
-            if constexpr (kMixY) {
+            if constexpr (kMixY && !LY) {
                 __builtin_memcpy(ynw, &cur.yn, sizeof(VT));
                 __builtin_memcpy(ysw, &cur.ys, sizeof(VT));
             }
@@ -840,8 +886,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                 // so the value is never used
                 PP[j] = S::ld(cur.pp.v[j]);
                 qv[j] = S::ld(cur.q.v[j]);
-                yn[j] = S::ld(cur.yn.v[j]);
-                ys[j] = S::ld(cur.ys.v[j]);
+                if constexpr (!LY) {
+                    yn[j] = S::ld(cur.yn.v[j]);
+                    ys[j] = S::ld(cur.ys.v[j]);
+                }
                 if constexpr (S::kScaled)
                     cd[j] = (cur.cd >> (4 * j)) & 0xFu;
                 else
@@ -856,9 +904,53 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
             }
             C left = __shfl_up(PC[V - 1], 1);
             C right = __shfl_down(PC[0], 1);
-            if (need_l) left = S::ld(cur.tl);
-            if (need_r) right = S::ld(cur.tr);
-            if (z + 1 < z1) issue(z + 1, nxt);
+            if constexpr (LY) {
+                // The last plane of a range issues its own plane again,
+                // unused: with a conditional issue the stage values would
+                // merge across the branch and be carried through the march,
+                // which took 40 more VGPRs (and a wave per SIMD).
+                issue(z + 1 < z1 ? z + 1 : z, nxt);
+                RawV *buf = ybuf[pcount & 1];
+                ++pcount;
+                VT own;
+#pragma unroll
+                for (int j = 0; j < V; ++j) own.v[j] = S::st(PC[j]); // exact: PC holds stored values
+                RawV r;
+                if (active) {
+                    __builtin_memcpy(&r, &own, sizeof(VT));
+                    buf[H + tid] = r;
+                }
+                if (valid_h) {
+                    __builtin_memcpy(&r, &cur.h, sizeof(VT));
+                    buf[hslot] = r;
+                }
+                __syncthreads();
+                // Lanes past the plane's last vector read slots of other
+                // data; their results are not stored, and in 16-bit storage
+                // their zero cell code makes each of them +0 or NaN, which
+                // the step maximum ignores.
+                const RawV rn = buf[2 * H + tid], rs = buf[tid];
+                VT ynv, ysv;
+                __builtin_memcpy(&ynv, &rn, sizeof(VT));
+                __builtin_memcpy(&ysv, &rs, sizeof(VT));
+                if constexpr (kMixY) {
+                    __builtin_memcpy(ynw, &ynv, sizeof(VT));
+                    __builtin_memcpy(ysw, &ysv, sizeof(VT));
+                }
+#pragma unroll
+                for (int j = 0; j < V; ++j) {
+                    yn[j] = S::ld(ynv.v[j]);
+                    ys[j] = S::ld(ysv.v[j]);
+                }
+                // the end lanes' x neighbours: cells of the adjacent vectors
+                const T *bt = reinterpret_cast<const T *>(buf);
+                if (need_l) left = S::ld(bt[(H + tid) * V - 1]);
+                if (need_r) right = S::ld(bt[(H + tid + 1) * V]);
+            } else {
+                if (need_l) left = S::ld(cur.tl);
+                if (need_r) right = S::ld(cur.tr);
+                if (z + 1 < z1) issue(z + 1, nxt);
+            }
 
             C o[V];
 #pragma unroll

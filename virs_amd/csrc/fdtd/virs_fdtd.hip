@@ -421,10 +421,10 @@ int dense_grid(const virs_ctx *c, K kernel, int &zranges)
     return (int)std::max<long long>(1, std::min<long long>(g, W));
 }
 
-template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE, bool SR, bool HARD = true>
-hipError_t launch_dense_k(const virs_ctx *c, const DenseParams &p)
+template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE, bool SR, bool HARD, bool LY>
+hipError_t launch_dense_kl(const virs_ctx *c, const DenseParams &p)
 {
-    auto k = k_dense<T, V, ADJ, FUSED, SPONGE, SR, HARD>;
+    auto k = k_dense<T, V, ADJ, FUSED, SPONGE, SR, HARD, LY>;
     DenseParams q = p;
     const int grid = dense_grid(c, k, q.zranges);
     if (c->kev[0])
@@ -432,6 +432,16 @@ hipError_t launch_dense_k(const virs_ctx *c, const DenseParams &p)
     else
         hipLaunchKernelGGL(k, dim3(grid), dim3(kBlock), 0, c->stream, q);
     return hipGetLastError();
+}
+
+// The stencil stages its planes through LDS when a row has at most
+// kHaloVecs vectors (k_dense LY), except with the fused boundary.
+template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE, bool SR, bool HARD = true>
+hipError_t launch_dense_k(const virs_ctx *c, const DenseParams &p)
+{
+    if constexpr (!FUSED)
+        if (c->nx / V <= kHaloVecs) return launch_dense_kl<T, V, ADJ, FUSED, SPONGE, SR, HARD, true>(c, p);
+    return launch_dense_kl<T, V, ADJ, FUSED, SPONGE, SR, HARD, false>(c, p);
 }
 
 template <typename T, int V, bool HARD>
