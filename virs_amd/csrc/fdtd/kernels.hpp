@@ -854,6 +854,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                 B0[j] = z0 >= 1 ? S::ld(tm.v[j]) : (C)0;
                 B1[j] = S::ld(tc.v[j]);
             }
+            // LY: a body stages P(z) from the other stage's pp, which holds
+            // it as stored until the body's issue reloads that stage
+            if constexpr (LY) SB.pp = tc;
         }
         issue(z0, SA);
 
@@ -905,21 +908,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
             C left = __shfl_up(PC[V - 1], 1);
             C right = __shfl_down(PC[0], 1);
             if constexpr (LY) {
+                RawV *buf = ybuf[pcount & 1];
+                ++pcount;
+                RawV r;
+                if (active) { // P(z) as stored (nxt.pp, see the prologue)
+                    __builtin_memcpy(&r, &nxt.pp, sizeof(VT));
+                    buf[H + tid] = r;
+                }
                 // The last plane of a range issues its own plane again,
                 // unused: with a conditional issue the stage values would
                 // merge across the branch and be carried through the march,
                 // which took 40 more VGPRs (and a wave per SIMD).
                 issue(z + 1 < z1 ? z + 1 : z, nxt);
-                RawV *buf = ybuf[pcount & 1];
-                ++pcount;
-                VT own;
-#pragma unroll
-                for (int j = 0; j < V; ++j) own.v[j] = S::st(PC[j]); // exact: PC holds stored values
-                RawV r;
-                if (active) {
-                    __builtin_memcpy(&r, &own, sizeof(VT));
-                    buf[H + tid] = r;
-                }
                 if (valid_h) {
                     __builtin_memcpy(&r, &cur.h, sizeof(VT));
                     buf[hslot] = r;
