@@ -211,15 +211,18 @@ def test_chunks_ending_mid_row(gpu, prec, nx):
     np.testing.assert_array_equal(outs[1][1], outs[0][1])
 
 
-@pytest.mark.parametrize("prec,nx", [("fp32", 384), ("fp32", 512), ("fp32", 516), ("fp16", 768),
-                                     ("fp16", 1024), ("bf16", 1032), ("fp64", 256), ("fp64", 258)])
+@pytest.mark.parametrize("prec,nx", [("fp32", 384), ("fp32", 512), ("fp32", 516), ("fp32", 1024),
+                                     ("fp32", 1028), ("fp16", 768), ("fp16", 1032), ("fp16", 2048),
+                                     ("fp16", 2056), ("bf16", 1536), ("fp64", 256), ("fp64", 258),
+                                     ("fp64", 514)])
 def test_rows_staged_through_lds_and_not(gpu, prec, nx):
-    """The stencil stages each plane through LDS when a row has at most 128
-    vectors (FP32 rows up to 512 cells, 16-bit up to 1024, FP64 up to 256)
-    and loads the rows +-Nx itself above that.  Both sides of the limit,
-    with chunks that end mid-row (Nx = 384 FP32, 768 FP16) and with several
-    chunks per block (a 7-block grid), give the oracle's results bit for
-    bit."""
+    """The stencil stages each plane through LDS with one halo vector per
+    thread when a row has at most 128 vectors (FP32 rows up to 512 cells,
+    16-bit up to 1024, FP64 up to 256), with two up to 256 vectors, and
+    loads the rows +-Nx itself above that.  Each side of both limits, with
+    chunks that end mid-row (Nx = 384 FP32, 768 FP16, 1536 BF16) and with
+    several chunks per block (a 7-block grid), gives the oracle's results
+    bit for bit."""
     p = tiny_problem(16, dims=(nx, 9, 10), steps=30)
     _say(f"rows staged or not {prec} nx={nx}")
     outs = []

@@ -21,8 +21,9 @@
 namespace virs {
 
 constexpr int kBlock = 256;   // 4 waves of 64
-// LDS staging of the stencil's planes (k_dense LY): rows of at most this
-// many vectors, so that a block's two halo rows take one vector per thread
+// LDS staging of the stencil's planes (k_dense LY): LY = 1 for rows of at
+// most kBlock / 2 vectors (a block's two halo rows take one vector per
+// thread), LY = 2 for rows of at most kBlock vectors (two per thread)
 constexpr int kHaloVecs = kBlock / 2;
 constexpr int kWaves = kBlock / 64;
 constexpr int kSlots = 64;    // spread of the per-step |p| maximum atomics
@@ -575,10 +576,11 @@ __device__ inline void store_df2(const BoundaryState &bs, long long e, long long
 // same plane.  The next plane's loads are issued before the current plane
 // is computed.  Every cell's update is computed and the result selected by
 // cell type, without branches.
-// LY (rows of at most kHaloVecs vectors): each plane is staged in LDS.  The
-// block writes its chunk of P(z), which its threads hold anyway, and one
-// row on either side (a halo vector per thread, loaded with the plane), then
-// reads the rows +-Nx and the end lanes' x neighbours back after a barrier.
+// LY (rows of at most LY * kHaloVecs vectors): each plane is staged in
+// LDS.  The block writes its chunk of P(z), which its threads hold anyway,
+// and one row on either side (LY halo vectors per thread, loaded with the
+// plane), then reads the rows +-Nx and the end lanes' x neighbours back
+// after a barrier.
 // Two buffers alternate by plane, so one barrier per plane orders both the
 // writes before the reads and the reads before the next overwrite.
 // Otherwise each thread loads its rows +-Nx itself (L2 hits: in tile mode
@@ -594,7 +596,7 @@ __device__ inline void store_df2(const BoundaryState &bs, long long e, long long
 // HARD (16-bit storage): the room has flag-3 cells.  Without them a wall or
 // border cell (P = Q = 0 there) gets the update with a zero Laplacian
 // factor, which is exactly +0, in place of two selects per cell.
-template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE, bool SR, bool HARD = true, bool LY = false>
+template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE, bool SR, bool HARD = true, int LY = 0>
 #ifndef VIRS_DENSE_WAVES
 #define VIRS_DENSE_WAVES 1 // minimum waves per SIMD the register allocation must allow
 #endif
@@ -694,7 +696,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
     using CD = typename std::conditional<S::kScaled, uint32_t, VB>::type;
     struct Stage {
         VT pp, q, yn, ys; // P(z+1), Q(z), P(z,y+1), P(z,y-1) as stored
-        VT h;             // LY: this thread's halo vector of P(z)
+        VT h, h2;         // LY: this thread's halo vectors of P(z)
         CD cd;            // cell codes of plane z
         T tl, tr;         // P(z) left of lane 0 / right of lane 63
         // FUSED: the wave's list slice at plane z (first entry, count; wave
@@ -724,9 +726,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
     // LY: two plane buffers of a chunk (slots 0..H-1 the row below it, H..
     // its own vectors, then the row above); the buffer alternates by plane
     static_assert(!(LY && FUSED), "the fused boundary keeps its own LDS use");
+    static_assert(LY >= 0 && LY <= 2, "one or two halo vectors per thread");
+    constexpr int kHalo = LY * kHaloVecs; // halo vectors per side at most
     using RawV = typename Raw<sizeof(VT)>::type;
-    __shared__ RawV ybuf[LY ? 2 : 1][LY ? kHaloVecs + kBlock + kHaloVecs : 1];
-    const int H = nx / V; // vectors per row; LY: at most kHaloVecs
+    __shared__ RawV ybuf[LY ? 2 : 1][LY ? kHalo + kBlock + kHalo : 1];
+    const int H = nx / V; // vectors per row; LY: at most kHalo
     unsigned pcount = 0;  // planes staged so far (LY)
     for (long long w = wbeg; w < wend;) {
         const int pchunk = (int)(w / nz);
@@ -759,31 +763,40 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
         // vector-offset form; otherwise the extension is hoisted out of the
         // march and every access builds a 64-bit address.  Rewriting the
         // loop-carried copies themselves needs no register copies.
-        // LY: thread t stages halo vector t of the chunk (t < 2H): vector
-        // v0 - H + t of the row below the chunk into slot t, or vector
-        // v0 + nv + t - H of the row above into slot nv + t; its own vector
-        // goes to slot H + t.  Halo vectors outside the plane are zero in
-        // both buffers for the whole chunk.
+        // LY: thread t stages halo vectors k = t and (LY = 2) k = t + kBlock
+        // of the chunk, k < 2H: vector v0 - H + k of the row below the chunk
+        // into slot k, or vector v0 + nv + k - H of the row above into slot
+        // nv + k; its own vector goes to slot H + t.  Halo vectors outside
+        // the plane are zero in both buffers for the whole chunk.
         const long long v0 = (long long)pchunk * kBlock;
         const int nv = (int)min((long long)kBlock, p.nvec_plane - v0);
-        const int tid = threadIdx.x;
+        const int tid = threadIdx.x, tid2 = tid + kBlock;
         const bool use_h = LY && tid < 2 * H;
         const long long hvec = tid < H ? v0 - H + tid : v0 + nv + (tid - H);
         const bool valid_h = use_h && hvec >= 0 && hvec < p.nvec_plane;
         const int hslot = tid < H ? tid : nv + tid;
+        const bool use_h2 = LY == 2 && tid2 < 2 * H; // then tid2 >= H: the row above
+        const long long hvec2 = v0 + nv + (tid2 - H);
+        const bool valid_h2 = use_h2 && hvec2 < p.nvec_plane;
+        const int hslot2 = nv + tid2;
         if constexpr (LY) {
             __syncthreads(); // the previous chunk's last reads are done
+            RawV zr;
+            __builtin_memset(&zr, 0, sizeof(RawV));
             if (use_h && !valid_h) {
-                RawV zr;
-                __builtin_memset(&zr, 0, sizeof(RawV));
                 ybuf[0][hslot] = zr;
                 ybuf[1][hslot] = zr;
             }
+            if (use_h2 && !valid_h2) {
+                ybuf[0][hslot2] = zr;
+                ybuf[1][hslot2] = zr;
+            }
         }
-        // LY: the halo vector's offset in a plane takes xon's place (a valid
-        // vector when there is nothing to load)
+        // LY: the halo vectors' offsets in a plane take xon's and xos's
+        // places (a valid vector when there is nothing to load)
         const uint32_t oh = (uint32_t)((valid_h ? hvec : v0) * V) * (uint32_t)sizeof(T);
-        uint32_t xoc = oc, xon = LY ? oh : on, xos = os, xob = S::kScaled ? ob >> 1 : ob;
+        const uint32_t oh2 = (uint32_t)((valid_h2 ? hvec2 : v0) * V) * (uint32_t)sizeof(T);
+        uint32_t xoc = oc, xon = LY ? oh : on, xos = LY ? oh2 : os, xob = S::kScaled ? ob >> 1 : ob;
         // stochastic-rounding hash: cell0's part of the pair index times K1
         const uint32_t c0k1 = ((uint32_t)cell0 & ~1u) * kHashK1;
         // FUSED: this wave's row of the slice table and its first cell in a plane
@@ -813,6 +826,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
             }
             if constexpr (LY) {
                 st.h = vload<VT, false>(reinterpret_cast<const VT *>(Pz + xon));
+                if constexpr (LY == 2) st.h2 = vload<VT, false>(reinterpret_cast<const VT *>(Pz + xos));
             } else {
                 st.yn = vload<VT, false>(reinterpret_cast<const VT *>(Pz + xon));
                 st.ys = vload<VT, false>(reinterpret_cast<const VT *>(Pz + xos));
@@ -924,6 +938,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                     __builtin_memcpy(&r, &cur.h, sizeof(VT));
                     buf[hslot] = r;
                 }
+                if constexpr (LY == 2)
+                    if (valid_h2) {
+                        __builtin_memcpy(&r, &cur.h2, sizeof(VT));
+                        buf[hslot2] = r;
+                    }
                 __syncthreads();
                 // Lanes past the plane's last vector read slots of other
                 // data; their results are not stored, and in 16-bit storage

@@ -421,7 +421,7 @@ int dense_grid(const virs_ctx *c, K kernel, int &zranges)
     return (int)std::max<long long>(1, std::min<long long>(g, W));
 }
 
-template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE, bool SR, bool HARD, bool LY>
+template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE, bool SR, bool HARD, int LY>
 hipError_t launch_dense_kl(const virs_ctx *c, const DenseParams &p)
 {
     auto k = k_dense<T, V, ADJ, FUSED, SPONGE, SR, HARD, LY>;
@@ -435,13 +435,16 @@ hipError_t launch_dense_kl(const virs_ctx *c, const DenseParams &p)
 }
 
 // The stencil stages its planes through LDS when a row has at most
-// kHaloVecs vectors (k_dense LY), except with the fused boundary.
+// 2 * kHaloVecs vectors (k_dense LY), except with the fused boundary.
 template <typename T, int V, bool ADJ, bool FUSED, bool SPONGE, bool SR, bool HARD = true>
 hipError_t launch_dense_k(const virs_ctx *c, const DenseParams &p)
 {
-    if constexpr (!FUSED)
-        if (c->nx / V <= kHaloVecs) return launch_dense_kl<T, V, ADJ, FUSED, SPONGE, SR, HARD, true>(c, p);
-    return launch_dense_kl<T, V, ADJ, FUSED, SPONGE, SR, HARD, false>(c, p);
+    if constexpr (!FUSED) {
+        const int h = c->nx / V;
+        if (h <= kHaloVecs) return launch_dense_kl<T, V, ADJ, FUSED, SPONGE, SR, HARD, 1>(c, p);
+        if (h <= 2 * kHaloVecs) return launch_dense_kl<T, V, ADJ, FUSED, SPONGE, SR, HARD, 2>(c, p);
+    }
+    return launch_dense_kl<T, V, ADJ, FUSED, SPONGE, SR, HARD, 0>(c, p);
 }
 
 template <typename T, int V, bool HARD>
