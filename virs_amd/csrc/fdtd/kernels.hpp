@@ -1311,22 +1311,35 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
 {
     using S = Store<T>;
     using C = typename S::C;
-    const long long n = p.ctl->nring[p.slot];
-    const bool skip = (n & kStopBit) != 0;
-    C sP = (C)1, sQ = (C)1, sR = (C)1, sRi = (C)1;
-    if constexpr (S::kScaled) {
-        // all four exponents in one load, selected by the step
-        const int4 ex = *(const int4 *)p.ctl->exp;
-        const int ring[4] = {ex.x, ex.y, ex.z, ex.w};
-        sP = exp2i(-ring[(n - 1) & 3]);
-        sQ = exp2i(-ring[(n - 2) & 3]);
-        const int eR = ring[n & 3];
-        sR = exp2i(eR);
-        sRi = exp2i(-eR);
-    }
+    // The step (with the stop bit) and, for 16-bit fields, the storage
+    // scales of p^n, p^{n-1} and p^{n+1}.  The list blocks read them after
+    // issuing their entries' loads, which do not depend on them, so the
+    // counter's load overlaps those instead of preceding them.
+    struct StepState {
+        long long n;
+        bool skip, in_sig, in_ir;
+        C sP, sQ, sR, sRi;
+    };
+    auto step_state = [&]() {
+        StepState st;
+        st.n = p.ctl->nring[p.slot];
+        st.skip = (st.n & kStopBit) != 0;
+        st.in_sig = st.n >= 0 && st.n < p.soft_len;
+        st.in_ir = st.n >= 0 && st.n < p.ir_cap;
+        st.sP = st.sQ = st.sR = st.sRi = (C)1;
+        if constexpr (S::kScaled) {
+            // all four exponents in one load, selected by the step
+            const int4 ex = *(const int4 *)p.ctl->exp;
+            const int ring[4] = {ex.x, ex.y, ex.z, ex.w};
+            st.sP = exp2i(-ring[(st.n - 1) & 3]);
+            st.sQ = exp2i(-ring[(st.n - 2) & 3]);
+            const int eR = ring[st.n & 3];
+            st.sR = exp2i(eR);
+            st.sRi = exp2i(-eR);
+        }
+        return st;
+    };
     T *__restrict__ R = (T *)p.R;
-    const bool in_sig = n >= 0 && n < p.soft_len;
-    const bool in_ir = n >= 0 && n < p.ir_cap;
     const long long nb = p.bs.nb;
     if (blockIdx.x > 0) { // list blocks exist only when nb > 0
         constexpr int U = kTailEntries;
@@ -1357,6 +1370,10 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
                 sb[u] = ((const float *)p.bs.ts[1])[ec];
             }
         }
+        const StepState ss = step_state();
+        const long long n = ss.n;
+        const bool skip = ss.skip, in_sig = ss.in_sig, in_ir = ss.in_ir;
+        const C sP = ss.sP, sQ = ss.sQ, sR = ss.sR, sRi = ss.sRi;
 #pragma unroll
         for (int u = 0; u < U; ++u) rt[u] = R[bv[u]];
         // before step n, tap slot n&1 holds step n-2's value and slot
@@ -1412,6 +1429,10 @@ __global__ __launch_bounds__(kTailBlock) void k_tail(const TailParams p)
         }
         return;
     }
+    const StepState ss = step_state();
+    const long long n = ss.n;
+    const bool in_sig = ss.in_sig, in_ir = ss.in_ir;
+    const C sR = ss.sR, sRi = ss.sRi;
     // Block 0: the points off the list, the RMS rule and the step counter.
     // Every table load is issued at once.  While a soft source is active,
     // its thread also writes the samples of the receivers on its cell, so
