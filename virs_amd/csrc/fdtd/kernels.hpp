@@ -622,8 +622,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
         else if (n >= 0 && n < p.hard_len) hard = (C)p.hard_sig[n];
     }
 
-    // kq: Q -> P units; kout: P units -> R's stored units; kphys: P units -> physical;
-    // kst / kst_inv: physical -> R's stored units and back
+    // kq: Q -> R's stored units; kout: P units -> R's stored units; kphys: P
+    // units -> physical; kst / kst_inv: physical -> R's stored units and back.
+    // The 16-bit update takes P in its stored units and Q in R's, with kout
+    // folded into its factors (2 kout, c2dt2 ih2 kout, the sponge's a kout),
+    // so it produces R's units directly: each is a power-of-two scaling of
+    // the same operation in physical units, which rounds alike.
     C kq = (C)1, kout = (C)1, kphys = (C)1, kst = (C)1, kst_inv = (C)1;
     C hard_u = hard;
     if constexpr (S::kScaled) {
@@ -640,12 +644,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
             eR = min(max(14 - k2, -60), 60);
         }
         const int eP = p.ctl->exp[ip], eQ = p.ctl->exp[iq];
-        kq = exp2i(eP - eQ);
+        kq = exp2i(eR - eQ);
         kout = exp2i(eR - eP);
         kphys = exp2i(-eP);
         kst = exp2i(eR);
         kst_inv = exp2i(-eR);
-        hard_u = hard * exp2i(eP);
+        hard_u = hard * exp2i(eR);
         if (bid == 0) {
             if (threadIdx.x == 0) {
                 p.ctl->exp[ir] = eR;
@@ -661,12 +665,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
     const long long sxy = p.sxy;
     const int nx = p.nx, ny = p.ny, nz = p.nz;
     const C c2dt2 = (C)p.c2dt2, ih2 = (C)p.ih2;
-    const C c2ih2 = c2dt2 * ih2; // 16-bit scheme: c2dt2*ih2 folded into one factor
+    const C c2ih2 = (c2dt2 * ih2) * kout; // 16-bit scheme: c2dt2*ih2 folded into one factor, in R's units
+    const C two_k = (C)2 * kout;
 
     __shared__ float sp_tab[SPONGE ? 3 : 1][SPONGE ? kMaxSponge : 1];
     if constexpr (SPONGE) {
         for (int k = threadIdx.x; k < p.sp.w; k += kBlock) {
-            sp_tab[0][k] = p.sp.a[k];
+            sp_tab[0][k] = S::kScaled ? p.sp.a[k] * (float)kout : p.sp.a[k];
             sp_tab[1][k] = p.sp.b[k];
             sp_tab[2][k] = p.sp.g[k];
         }
@@ -913,7 +918,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                     cd[j] = (uint32_t)cur.cd.v[j];
             }
             if constexpr (S::kScaled) {
-                // Q to P units; consecutive steps mostly share an exponent,
+                // Q to R's units; consecutive steps mostly share an exponent,
                 // and a multiply by one is exact, so it is skipped then
                 if (kq != (C)1)
 #pragma unroll
@@ -1004,7 +1009,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                     // step drove the room's constant mode (relative IR error
                     // 0.36 against 5e-3 on the 2000-step shoebox).
                     const C t = __builtin_fmaf(-kw, pcv, sum);
-                    C a = (C)2, b = (C)1, g = (C)1;
+                    C a = two_k, b = (C)1, g = (C)1;
                     if constexpr (SPONGE) {
                         const int d = edge_dist(x0 + j, y, z, nx, ny, nz);
                         if (d < p.sp.w) {
@@ -1021,12 +1026,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                         if constexpr (SPONGE)
                             upd = __builtin_fmaf(t, g * kf, __builtin_fmaf(a, pcv, -(b * qv[j])));
                         else
-                            upd = __builtin_fmaf(t, kf, __builtin_fmaf((C)2, pcv, -qv[j]));
+                            upd = __builtin_fmaf(t, kf, __builtin_fmaf(two_k, pcv, -qv[j]));
                     } else {
                         if constexpr (SPONGE)
                             upd = (a * pcv - b * qv[j]) + g * (t * kf);
                         else
-                            upd = __builtin_fmaf((C)2, pcv, -qv[j]) + t * kf;
+                            upd = __builtin_fmaf(two_k, pcv, -qv[j]) + t * kf;
                     }
                     o[j] = HARD ? bsel(bit_mask(c, 3), bsel(bit_mask(c, 0), hard_u, (C)0), upd) : upd;
                     continue;
@@ -1063,9 +1068,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
             if constexpr (S::kScaled) {
 #pragma unroll
                 for (int j = 0; j < V; ++j) lmax = fmaxf(lmax, fabsf((float)o[j]));
-                if (kout != (C)1) // P units to R's stored units (skipped when exact)
-#pragma unroll
-                    for (int j = 0; j < V; ++j) o[j] = o[j] * kout;
                 if constexpr (SR) {
                     // the hash of mix16 for cell i = z*sxy + cell0 in 32-bit
                     // pieces: hk takes the high word of i (the plane's plus a
@@ -1116,7 +1118,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                 if (cnt != 0u) {
 #pragma unroll
                     for (int j = 0; j < V; ++j) {
-                        const C d = S::kScaled ? PC[j] * kphys - qv[j] * kphys : PC[j] - qv[j];
+                        const C d = S::kScaled ? PC[j] * kphys - qv[j] * kst_inv : PC[j] - qv[j];
                         bslot[wave][j][lane] = BSlot{d, S::ld(out.v[j])};
                     }
                     wave_lds_fence();
@@ -1189,7 +1191,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
 
     if constexpr (S::kScaled) {
         __shared__ float red[kWaves];
-        lmax = wave_max(lmax) * (float)kphys; // P units -> physical
+        lmax = wave_max(lmax) * (float)kst_inv; // R's units -> physical
         if (lane == 0) red[wave] = lmax;
         __syncthreads();
         if (threadIdx.x == 0) {
