@@ -585,9 +585,11 @@ __device__ inline void store_df2(const BoundaryState &bs, long long e, long long
 // writes before the reads and the reads before the next overwrite.
 // Otherwise each thread loads its rows +-Nx itself (L2 hits: in tile mode
 // the neighbouring chunks march the same z together).
-// 16-bit storage: arithmetic runs in p^n's stored units (value * 2^eP);
-// power-of-two scaling commutes exactly with every rounding step, so the
-// results equal a computation in physical units bit for bit.  Walls hold
+// 16-bit storage: arithmetic runs on the stored values, p^n in its units
+// (value * 2^eP) and q and the result in p^{n+1}'s (2^eR, folded into the
+// update's factors); power-of-two scaling commutes exactly with every
+// rounding step, so the results equal a computation in physical units bit
+// for bit.  Walls hold
 // zero and the code carries the centre weight kw (k_build_code16); SR
 // selects stochastic storage rounding.
 // FUSED: the boundary biquad of the list entries runs inside the march (see
