@@ -254,6 +254,19 @@ __device__ inline float add_f16_half(float sum, uint32_t w)
     return r;
 }
 
+// a * h + c rounded once, h the FP16 half of w (high half if HI), with a
+// and c FP32
+template <bool HI>
+__device__ inline float fma_f16_half(float a, uint32_t w, float c)
+{
+    float r;
+    if constexpr (HI)
+        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[0,1,0] op_sel_hi:[0,1,0]" : "=v"(r) : "v"(a), "v"(w), "v"(c));
+    else
+        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,1,0]" : "=v"(r) : "v"(a), "v"(w), "v"(c));
+    return r;
+}
+
 // Orders a wave's LDS accesses around a lane-to-lane exchange inside the
 // wave: the hardware performs one wave's LDS operations in issue order, so
 // only the compiler's ordering needs fixing, which a wavefront-scope fence
@@ -669,6 +682,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
     const C c2dt2 = (C)p.c2dt2, ih2 = (C)p.ih2;
     const C c2ih2 = (c2dt2 * ih2) * kout; // 16-bit scheme: c2dt2*ih2 folded into one factor, in R's units
     const C two_k = (C)2 * kout;
+    const C nkq = -kq;
 
     __shared__ float sp_tab[SPONGE ? 3 : 1][SPONGE ? kMaxSponge : 1];
     if constexpr (SPONGE) {
@@ -904,12 +918,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                 __builtin_memcpy(ynw, &cur.yn, sizeof(VT));
                 __builtin_memcpy(ysw, &cur.ys, sizeof(VT));
             }
+            // FP16 without a sponge: q enters 2 pc - q straight from its
+            // stored pairs as one mixed fused multiply-add by -kq (exact
+            // products, one rounding: the bits of the rescale-then-subtract
+            // form), so neither a conversion nor a rescale step is needed
+            constexpr bool kMixQ = std::is_same<T, f16>::value && V >= 2 && !SPONGE && !FUSED;
+            uint32_t qw[kMixQ ? V / 2 : 1];
+            if constexpr (kMixQ) __builtin_memcpy(qw, &cur.q, sizeof(VT));
 #pragma unroll
             for (int j = 0; j < V; ++j) {
                 // at z = nz-1 this is plane z itself; that plane is border,
                 // so the value is never used
                 PP[j] = S::ld(cur.pp.v[j]);
-                qv[j] = S::ld(cur.q.v[j]);
+                if constexpr (!kMixQ) qv[j] = S::ld(cur.q.v[j]);
                 if constexpr (!LY) {
                     yn[j] = S::ld(cur.yn.v[j]);
                     ys[j] = S::ld(cur.ys.v[j]);
@@ -919,7 +940,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                 else
                     cd[j] = (uint32_t)cur.cd.v[j];
             }
-            if constexpr (S::kScaled) {
+            if constexpr (S::kScaled && !kMixQ) {
                 // Q to R's units; consecutive steps mostly share an exponent,
                 // and a multiply by one is exact, so it is skipped then
                 if (kq != (C)1)
@@ -1023,17 +1044,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(VIRS_DEN
                     // Laplacian factor: zero on zero cells when the room has
                     // no hard cells (a*0 - b*0 and t*0 then give +0)
                     const C kf = HARD ? c2ih2 : bsel(bit_mask(c, 3), (C)0, c2ih2);
-                    C upd;
+                    C upd, u2 = (C)0; // u2 = 2 pc - q (no sponge)
+                    if constexpr (kMixQ)
+                        u2 = (j & 1) ? fma_f16_half<true>(nkq, qw[j >> 1], two_k * pcv)
+                                     : fma_f16_half<false>(nkq, qw[j >> 1], two_k * pcv);
+                    else if constexpr (!SPONGE)
+                        u2 = __builtin_fmaf(two_k, pcv, -qv[j]);
                     if constexpr (SR) {
                         if constexpr (SPONGE)
                             upd = __builtin_fmaf(t, g * kf, __builtin_fmaf(a, pcv, -(b * qv[j])));
                         else
-                            upd = __builtin_fmaf(t, kf, __builtin_fmaf(two_k, pcv, -qv[j]));
+                            upd = __builtin_fmaf(t, kf, u2);
                     } else {
                         if constexpr (SPONGE)
                             upd = (a * pcv - b * qv[j]) + g * (t * kf);
                         else
-                            upd = __builtin_fmaf(two_k, pcv, -qv[j]) + t * kf;
+                            upd = u2 + t * kf;
                     }
                     o[j] = HARD ? bsel(bit_mask(c, 3), bsel(bit_mask(c, 0), hard_u, (C)0), upd) : upd;
                     continue;
